@@ -1,0 +1,239 @@
+/*
+ * simulon_hip.h — C ABI of the MI355X (gfx950) MD-step library libsimulon_hip.so.
+ *
+ * Every entry point takes raw device pointers, element counts and a
+ * hipStream_t (passed as void*), launches stream-ordered work only (no
+ * device-wide synchronisation, no allocation), and returns 0 on success or a
+ * negative SIM_E* code.  No entry point keeps global state: two calls on
+ * different data may run concurrently on different streams.
+ *
+ * State is packed for R >= 1 independent systems ("replicas"): atoms of
+ * replica r occupy [atom_base, atom_base + natoms) of every per-atom array,
+ * and a single system is simply R = 1.  Atom-parallel kernels walk a tile
+ * table (replica, first atom) with SIM_TILE atoms per tile so that each
+ * workgroup reads its replica's box and parameters as uniform scalars.
+ *
+ * Per-atom arrays:
+ *   pos4   double[N][4]  x, y, z (unwrapped, Angstrom) and the atom type as a double
+ *   vel    float [N][3]  velocity, Angstrom/ps
+ *   force  float [N][3]  eV/Angstrom
+ *   nbr    int32 [max_nbr][N]  full Verlet list, column-major (entry k of atom i
+ *                              at nbr[k*N + i]), global atom indices
+ *   nnbr   int32 [N]
+ *
+ * Reference interfaces replaced (JinZongxiao/Simulon):
+ *   sim_nl_build / sim_nl_half_*   simulon_cuda.neighbor_search_cuda
+ *        ("cuda source/neighbor_search_kernel.cu:73-138") and the PyTorch
+ *        brute force core/neighbor_search/gpu_kdtree.py:90-156
+ *   sim_nl_trigger                 skin/2 test of io_utils/reader.py:136-149
+ *   sim_lj_force                   simulon_cuda.lj_energy_force_cuda
+ *        ("cuda source/lj_energy_force.cpp:10-20") + configure_lj_smoothing
+ *        ("cuda source/ops.cpp:39-45") and core/force/lennard_jones_force.py:15-95
+ *   sim_eam_density / sim_eam_force   simulon_cuda.density_pass / force_pass
+ *        ("cuda source/eam_cuda_ext.cpp:30-60") and the rest of
+ *        core/force/eam_force_cu.py:461-579
+ *   sim_integrate_first / _second  core/integrator/integrator.py:91-124
+ *   sim_reduce                     the torch .sum()/einsum reductions of the
+ *        force fields (e.g. core/force/lennard_jones_force.py:92-94),
+ *        core/md_model.py:83-86 and integrator.py:118-119
+ *   sim_berendsen                  core/barostat.py:36-68
+ */
+#ifndef SIMULON_HIP_H
+#define SIMULON_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIM_OK 0
+#define SIM_E_ARG (-1)       /* bad argument (null pointer, size out of range) */
+#define SIM_E_LAUNCH (-2)    /* kernel launch failed */
+#define SIM_E_TYPES (-3)     /* too many atom types */
+
+#define SIM_TILE 256         /* atoms per tile = threads per workgroup */
+#define SIM_MAX_TYPES 4
+#define SIM_NPART 16         /* doubles per tile in the partial-sum array */
+
+/* Columns of the partial-sum and per-replica thermo arrays (double[16]). */
+#define SIM_T_EPOT 0         /* potential energy, eV */
+#define SIM_T_VIRIAL 1       /* scalar virial with the force field's own sign */
+#define SIM_T_W 2            /* virial tensor sum f (x) r: xx yy zz xy xz yz (2..7) */
+#define SIM_T_KE 8           /* kinetic energy 1/2 sum m v^2 */
+#define SIM_T_K 9            /* kinetic tensor sum m v_a v_b: xx yy zz xy xz yz (9..14) */
+#define SIM_T_MAXD2 15       /* (sim_reduce output) unused */
+
+/* Per-replica state, one struct per replica in device memory. */
+typedef struct sim_replica {
+    double h[9];          /* lattice vectors as rows, row-major */
+    double hinv[9];
+    double volume;
+    double rl2;           /* squared Verlet list radius (value of fp32 (rc+skin)^2) */
+    double trig2;         /* squared rebuild displacement (skin/2)^2 */
+    double rl;            /* list radius */
+    double skin;
+    double mu_acc;        /* isotropic box scaling accumulated since the last build */
+    double dt;            /* time step, ps */
+    double mu;            /* last Berendsen scaling factor */
+    double press;         /* last instantaneous pressure, bar */
+    double target_p;      /* Berendsen target pressure, bar */
+    double tau_p;         /* Berendsen time constant, ps */
+    double kappa;         /* compressibility, 1/bar */
+    double mu_max;
+    float damp;           /* Langevin exp(-gamma dt) */
+    float rff[SIM_MAX_TYPES]; /* Langevin kick std per type, Angstrom/ps */
+    float rcf;            /* force cutoff */
+    float dtf;            /* time step as float (reference arithmetic) */
+    int atom_base, natoms;
+    int cell_base, cell_cap;
+    int ncell[3];
+    int ortho;
+    int langevin;         /* 1: Langevin O step in the second half */
+    int barostat;         /* 1: Berendsen after the second half */
+    int pad;
+} sim_replica_t;
+
+typedef struct sim_types {
+    float mass[SIM_MAX_TYPES];   /* internal mass eV ps^2 / A^2, per type */
+    int ntypes;
+    int pad;
+} sim_types_t;
+
+typedef struct sim_lj_params {
+    float eps[SIM_MAX_TYPES * SIM_MAX_TYPES];
+    float sig[SIM_MAX_TYPES * SIM_MAX_TYPES];
+    float rc;        /* cutoff */
+    float rs;        /* switch start (mode 2) */
+    int mode;        /* 0 raw, 1 force-shift, 2 polynomial switch */
+    int pad;
+} sim_lj_params_t;
+
+typedef struct sim_eam_params {
+    const void* pass1;    /* float4 [nt][nt][n_r]: f(k), f(k+1), phi(k), phi(k+1) */
+    const void* pass2;    /* float4 [nt][nt][n_r]: f'(k), f'(k+1), phi'(k), phi'(k+1) */
+    const float* emb;     /* float2 [nt][n_rho]: F(k), F(k+1) */
+    const float* demb;    /* float2 [nt][n_rho]: F'(k), F'(k+1) */
+    int n_r;
+    int n_rho;
+    int ntypes;
+    int pad;
+    float inv_dr;         /* float(1/dr) */
+    float r_clamp;        /* float(r_max (1 - 1e-7)) */
+    float cutoff;         /* r < cutoff counts */
+    float rho_lo[SIM_MAX_TYPES];
+    float rho_hi[SIM_MAX_TYPES];   /* float(lo + drho (n-1)(1 - 1e-7)) */
+    float inv_drho[SIM_MAX_TYPES];
+} sim_eam_params_t;
+
+/* Library identification: writes "simulon_hip <version> gfx950". */
+int sim_version(char* buf, int len);
+
+/* ---- neighbor list -------------------------------------------------- */
+
+/* Full Verlet list build for every replica whose rebuild flag is set
+ * (flag == NULL: all replicas).  Cell grid, counting sort by cell, an
+ * exclusive scan over all cells, a per-cell index sort (deterministic order)
+ * and the list itself.  Also copies pos4 into pos_last (the reference of the
+ * displacement trigger) and resets mu_acc.
+ *   cell_rep: int[C] replica of each cell slot (C = total cell capacity)
+ *   atom_cell, atom_slot: int[N] work arrays; cell_count/cell_start: int[C+1];
+ *   cell_atoms: int[N]; overflow: int[1], set to 1 when a list exceeds max_nbr
+ *   n: total atoms (the list's column stride); tiles: int2 (replica, first atom)
+ *   precise: 1 = pair test in float64, 0 = in float32 (reference fp32 semantics) */
+int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
+                 const double* pos4, double* pos_last, const int* flag,
+                 const int* cell_rep, int ncells,
+                 int* atom_cell, int* atom_slot, int* cell_count, int* cell_start,
+                 int* cell_atoms, int* nbr, int* nnbr, int max_nbr, int* overflow,
+                 int precise, void* stream);
+
+/* Same build for a single system given as plain [N,3] coordinates, float32
+ * (dtype 0) or float64 (dtype 1): the pair test uses exactly the reference's
+ * arithmetic in that dtype (core/box.py:122-130, gpu_kdtree.py:130-138).
+ * rep: one sim_replica_t in device memory (box, rl2, cell grid). */
+int sim_nl_build_xyz(void* rep, const void* xyz, int dtype, int n,
+                     int* atom_cell, int* atom_slot, int* cell_count, int* cell_start,
+                     int* cell_atoms, int ncells, int* nbr, int* nnbr, int max_nbr,
+                     int* overflow, void* stream);
+
+/* Half list (i < j) of a full list: per-atom counts of j > i and their
+ * exclusive prefix (int[N+1], last entry = E) ... */
+int sim_nl_half_count(const int* nbr, const int* nnbr, int n, int max_nbr,
+                      int* half_count, int* half_start, void* stream);
+/* ... then edge_index int64 [2][E] in lexicographic order and the distances
+ * |minimum_image(x_i - x_j)| in the coordinate dtype. */
+int sim_nl_half_fill(void* rep, const void* xyz, int dtype, int n,
+                     const int* nbr, const int* nnbr, int max_nbr,
+                     const int* half_start, long long* edge_index, long long E,
+                     void* dist, void* stream);
+
+/* Skin trigger: flag[r] = force_all || (max displacement over replica r)^2 >
+ * threshold, where the threshold accounts for box scaling since the build.
+ * maxd2: double[R] running maxima written by sim_integrate_first (reset here);
+ * nbuild: optional int[R] rebuild counters (incremented when flag is set). */
+int sim_nl_trigger(int R, void* reps, double* maxd2, int* flag, int* nbuild, int force_all,
+                   void* stream);
+
+/* ---- force fields ----------------------------------------------------- */
+
+/* LJ energy/force/virial over the full list; writes force[N][3] and the
+ * SIM_T_EPOT..SIM_T_W columns of partial[ntiles][16]. virial: 0 skips the
+ * per-pair virial (columns written as 0). */
+int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
+                 const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                 sim_lj_params_t lj, float* force, double* partial, int virial,
+                 void* stream);
+
+/* EAM pass 1: rho_i, F(rho_i), F'(rho_i) (written to fp[N]) and the energy
+ * column; pass 2: forces and virial columns. */
+int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
+                    const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                    sim_eam_params_t eam, float* rho, float* fp, double* partial,
+                    void* stream);
+int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
+                  const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                  sim_eam_params_t eam, const float* fp, float* force, double* partial,
+                  int virial, void* stream);
+
+/* ---- integrator --------------------------------------------------------- */
+
+/* First half (B + A): vh = v + 1/2 F/m dt (float32), x += vh dt (float64),
+ * per-replica max |x - pos_last|^2 into maxd2 (double bits, atomic max). */
+int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
+                        sim_types_t types, double* pos4, const float* vel,
+                        const float* force, float* vel_half, const double* pos_last,
+                        double* maxd2, void* stream);
+
+/* Second half (O + B): v = vh damp + xi rff (Langevin replicas), then
+ * v += 1/2 F/m dt; KE and kinetic-tensor columns of partial.
+ * noise: optional float[N][3] of N(0,1) draws (NULL = Philox4x32-10 keyed by
+ * seed, counter = (global atom index, step)). */
+int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
+                         sim_types_t types, const double* pos4, const float* vel_half,
+                         const float* force, float* vel, const float* noise,
+                         unsigned long long seed, unsigned long long step,
+                         const unsigned long long* step_dev, double* partial, void* stream);
+
+/* step_dev[0] += 1 (device step counter read by sim_integrate_second, so that
+ * a captured step graph draws fresh Langevin noise on every replay). */
+int sim_step_advance(unsigned long long* step_dev, void* stream);
+
+/* Per-replica compensated FP64 sums of partial columns [col0, col1) over the
+ * replica's tiles (rep_tiles: int[R+1]) into thermo[R][16]. */
+int sim_reduce(int R, const int* rep_tiles, const double* partial, int col0, int col1,
+               double* thermo, void* stream);
+
+/* Berendsen (barostat.py:36-68) for replicas with barostat = 1: pressure from
+ * thermo K + W and the volume, mu, box update in reps; then x, v and
+ * pos_last scaled by mu. */
+int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double* thermo,
+                  double* pos4, float* vel, double* pos_last, void* stream);
+
+/* Copies positions into a float32 [N,3] (dtype 0) or float64 [N,3] (dtype 1)
+ * array, and back (sim_pos_import). */
+int sim_pos_export(const double* pos4, int n, void* xyz, int dtype, void* stream);
+int sim_pos_import(double* pos4, int n, const void* xyz, int dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,352 @@
+// Fused velocity-Verlet / Langevin / Berendsen update, skin trigger and the
+// float64 per-replica reductions (gfx950).
+//
+// core/integrator/integrator.py:91-124 splits a step into B+A (first half)
+// and O+B (second half); each half is one kernel here. Velocities stay float32
+// with the reference's arithmetic (a = F/m, v + (0.5 a) dt); positions are
+// float64 and advanced as x += double(v_half) dt. The first half also measures
+// the displacement since the last list build (io_utils/reader.py:136-149) and
+// folds it into a per-replica maximum; the Langevin kick uses Philox4x32-10
+// with Box-Muller, keyed by (seed, step, global atom index).
+#include "sim_common.h"
+
+namespace {
+
+__device__ __forceinline__ SimTile tile_of(const SimTile* tiles) { return tiles[blockIdx.x]; }
+
+__global__ void __launch_bounds__(SIM_TILE)
+    k_first(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+            sim_types_t ty, double4* __restrict__ pos, const float* __restrict__ vel,
+            const float* __restrict__ force, float* __restrict__ vh_out,
+            const double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2) {
+    __shared__ double lds[4];
+    const SimTile t = tile_of(tiles);
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    double d2 = 0.0;
+    if (i < P.atom_base + P.natoms) {
+        double4 x = pos[i];
+        const float m = ty.mass[(int)x.w];
+        const float dt = P.dtf;
+        const size_t o = (size_t)i * 3;
+        const float v0 = vel[o] + (0.5f * (force[o] / m)) * dt;
+        const float v1 = vel[o + 1] + (0.5f * (force[o + 1] / m)) * dt;
+        const float v2 = vel[o + 2] + (0.5f * (force[o + 2] / m)) * dt;
+        vh_out[o] = v0;
+        vh_out[o + 1] = v1;
+        vh_out[o + 2] = v2;
+        x.x += (double)v0 * P.dt;
+        x.y += (double)v1 * P.dt;
+        x.z += (double)v2 * P.dt;
+        pos[i] = x;
+        const double4 l = pos_last[i];
+        const double ex = x.x - l.x, ey = x.y - l.y, ez = x.z - l.z;
+        d2 = (ex * ex + ey * ey) + ez * ez;
+    }
+    d2 = wave_max(d2);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+        // non-negative doubles order like their bit patterns
+        atomicMax(&maxd2[t.rep], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void __launch_bounds__(SIM_TILE)
+    k_second(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+             sim_types_t ty, const double4* __restrict__ pos, const float* __restrict__ vh,
+             const float* __restrict__ force, float* __restrict__ vel,
+             const float* __restrict__ noise, unsigned long long seed, unsigned long long step0,
+             const unsigned long long* __restrict__ step_dev, double* __restrict__ partial) {
+    const unsigned long long step = step0 + (step_dev ? step_dev[0] : 0ull);
+    __shared__ double lds[4 * 7];
+    const SimTile t = tile_of(tiles);
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    double ke = 0.0, kxx = 0.0, kyy = 0.0, kzz = 0.0, kxy = 0.0, kxz = 0.0, kyz = 0.0;
+    if (i < P.atom_base + P.natoms) {
+        const int ty_i = (int)pos[i].w;
+        const float m = ty.mass[ty_i];
+        const float dt = P.dtf;
+        const size_t o = (size_t)i * 3;
+        float v0 = vh[o], v1 = vh[o + 1], v2 = vh[o + 2];
+        if (P.langevin) {
+            float g0, g1, g2;
+            if (noise) {
+                g0 = noise[o];
+                g1 = noise[o + 1];
+                g2 = noise[o + 2];
+            } else {
+                gauss3(seed, step, (uint32_t)i, g0, g1, g2);
+            }
+            const float s = P.rff[ty_i];
+            v0 = v0 * P.damp + g0 * s;
+            v1 = v1 * P.damp + g1 * s;
+            v2 = v2 * P.damp + g2 * s;
+        }
+        v0 = v0 + (0.5f * (force[o] / m)) * dt;
+        v1 = v1 + (0.5f * (force[o + 1] / m)) * dt;
+        v2 = v2 + (0.5f * (force[o + 2] / m)) * dt;
+        vel[o] = v0;
+        vel[o + 1] = v1;
+        vel[o + 2] = v2;
+        const double md = (double)m, a = v0, b = v1, c = v2;
+        ke = 0.5 * md * ((a * a + b * b) + c * c);
+        kxx = md * a * a;
+        kyy = md * b * b;
+        kzz = md * c * c;
+        kxy = md * a * b;
+        kxz = md * a * c;
+        kyz = md * b * c;
+    }
+    double v[7] = {ke, kxx, kyy, kzz, kxy, kxz, kyz};
+    block_sum<7>(v, lds);
+    if (threadIdx.x == 0) {
+        double* p = partial + (size_t)blockIdx.x * SIM_NPART;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) p[SIM_T_KE + k] = v[k];
+    }
+}
+
+__global__ void k_step_advance(unsigned long long* c) { c[0] += 1ull; }
+
+__global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2, int* flag,
+                          int* nbuild, int force_all) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const sim_replica_t& P = reps[r];
+    const double d2 = __longlong_as_double((long long)maxd2[r]);
+    // pairs closer than r_l at the build stay inside the list while
+    // 2 max|dx| <= skin - max(0, 1 - mu_acc) r_l  (isotropic box scaling)
+    const double allow = 0.5 * (P.skin - fmax(0.0, 1.0 - P.mu_acc) * P.rl);
+    const bool trip = allow <= 0.0 || d2 > (P.mu_acc == 1.0 ? P.trig2 : allow * allow);
+    const int f = (force_all || trip) ? 1 : 0;
+    flag[r] = f;
+    if (nbuild) nbuild[r] += f;
+    maxd2[r] = 0ull;
+}
+
+// Per-replica compensated (Neumaier) sums of partial columns, one workgroup
+// per replica.
+__global__ void __launch_bounds__(256)
+    k_reduce(const int* __restrict__ rep_tiles, const double* __restrict__ partial, int c0,
+             int c1, double* __restrict__ thermo) {
+    __shared__ double ls[256], lc[256];
+    const int r = blockIdx.x;
+    const int b = rep_tiles[r], e = rep_tiles[r + 1];
+    for (int c = c0; c < c1; ++c) {
+        double s = 0.0, comp = 0.0;
+        for (int k = b + (int)threadIdx.x; k < e; k += 256) {
+            const double x = partial[(size_t)k * SIM_NPART + c];
+            const double tt = s + x;
+            comp += fabs(s) >= fabs(x) ? (s - tt) + x : (x - tt) + s;
+            s = tt;
+        }
+        ls[threadIdx.x] = s;
+        lc[threadIdx.x] = comp;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                const double x = ls[threadIdx.x + o];
+                const double a = ls[threadIdx.x];
+                const double tt = a + x;
+                const double cc = fabs(a) >= fabs(x) ? (a - tt) + x : (x - tt) + a;
+                ls[threadIdx.x] = tt;
+                lc[threadIdx.x] += lc[threadIdx.x + o] + cc;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) thermo[(size_t)r * SIM_NPART + c] = ls[0] + lc[0];
+        __syncthreads();
+    }
+}
+
+// Berendsen, core/barostat.py:46-66 (pressure from K + W, mu^3 clamp, box and
+// coordinates scaled); pos_last is scaled too so that the rebuild trigger keeps
+// measuring displacements in the co-moving frame.
+__global__ void k_baro_box(int R, sim_replica_t* reps, const double* thermo) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    sim_replica_t& P = reps[r];
+    if (!P.barostat) {
+        P.mu = 1.0;
+        return;
+    }
+    const double* th = thermo + (size_t)r * SIM_NPART;
+    const double tr = (th[SIM_T_K] + th[SIM_T_K + 1] + th[SIM_T_K + 2]) +
+                      (th[SIM_T_W] + th[SIM_T_W + 1] + th[SIM_T_W + 2]);
+    const double Pbar = tr / P.volume / 3.0 * 1602176.6208;
+    double mu3 = 1.0 + P.kappa * (P.dt / P.tau_p) * (Pbar - P.target_p);
+    const double lo = (1.0 - P.mu_max) * (1.0 - P.mu_max) * (1.0 - P.mu_max);
+    const double hi = (1.0 + P.mu_max) * (1.0 + P.mu_max) * (1.0 + P.mu_max);
+    mu3 = fmin(fmax(mu3, lo), hi);
+    const double mu = pow(mu3, 1.0 / 3.0);
+    for (int k = 0; k < 9; ++k) {
+        P.h[k] *= mu;
+        P.hinv[k] /= mu;
+    }
+    P.volume *= mu3 == 0.0 ? 0.0 : mu * mu * mu;
+    P.mu = mu;
+    P.mu_acc *= mu;
+    P.press = Pbar;
+}
+
+__global__ void __launch_bounds__(SIM_TILE)
+    k_baro_scale(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+                 double4* __restrict__ pos, float* __restrict__ vel, double4* __restrict__ pos_last) {
+    const SimTile t = tile_of(tiles);
+    const sim_replica_t& P = reps[t.rep];
+    if (!P.barostat) return;
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const double mu = P.mu;
+    const float muf = (float)mu;
+    double4 x = pos[i];
+    x.x *= mu;
+    x.y *= mu;
+    x.z *= mu;
+    pos[i] = x;
+    double4 l = pos_last[i];
+    l.x *= mu;
+    l.y *= mu;
+    l.z *= mu;
+    pos_last[i] = l;
+    const size_t o = (size_t)i * 3;
+    vel[o] *= muf;
+    vel[o + 1] *= muf;
+    vel[o + 2] *= muf;
+}
+
+template <typename T>
+__global__ void k_export(const double4* pos, int n, T* xyz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 x = pos[i];
+    xyz[(size_t)i * 3] = (T)x.x;
+    xyz[(size_t)i * 3 + 1] = (T)x.y;
+    xyz[(size_t)i * 3 + 2] = (T)x.z;
+}
+
+template <typename T>
+__global__ void k_import(double4* pos, int n, const T* xyz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 x = pos[i];
+    x.x = (double)xyz[(size_t)i * 3];
+    x.y = (double)xyz[(size_t)i * 3 + 1];
+    x.z = (double)xyz[(size_t)i * 3 + 2];
+    pos[i] = x;
+}
+
+}  // namespace
+
+extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
+                                   sim_types_t types, double* pos4, const float* vel,
+                                   const float* force, float* vel_half, const double* pos_last,
+                                   double* maxd2, void* stream) {
+    if (!reps || !tiles || ntiles <= 0 || !pos4 || !vel || !force || !vel_half || !pos_last ||
+        !maxd2 || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES)
+        return SIM_E_ARG;
+    hipLaunchKernelGGL(k_first, dim3(ntiles), dim3(SIM_TILE), 0, as_stream(stream),
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles), types,
+                       reinterpret_cast<double4*>(pos4), vel, force, vel_half,
+                       reinterpret_cast<const double4*>(pos_last),
+                       reinterpret_cast<unsigned long long*>(maxd2));
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
+                                    sim_types_t types, const double* pos4, const float* vel_half,
+                                    const float* force, float* vel, const float* noise,
+                                    unsigned long long seed, unsigned long long step,
+                                    const unsigned long long* step_dev, double* partial,
+                                    void* stream) {
+    if (!reps || !tiles || ntiles <= 0 || !pos4 || !vel_half || !force || !vel || !partial ||
+        types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES)
+        return SIM_E_ARG;
+    hipLaunchKernelGGL(k_second, dim3(ntiles), dim3(SIM_TILE), 0, as_stream(stream),
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles), types,
+                       reinterpret_cast<const double4*>(pos4), vel_half, force, vel, noise, seed,
+                       step, step_dev, partial);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_step_advance(unsigned long long* step_dev, void* stream) {
+    if (!step_dev) return SIM_E_ARG;
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, as_stream(stream), step_dev);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_nl_trigger(int R, void* reps, double* maxd2, int* flag, int* nbuild,
+                              int force_all, void* stream) {
+    if (R <= 0 || !reps || !maxd2 || !flag) return SIM_E_ARG;
+    hipLaunchKernelGGL(k_trigger, dim3((R + 63) / 64), dim3(64), 0, as_stream(stream), R,
+                       static_cast<sim_replica_t*>(reps),
+                       reinterpret_cast<unsigned long long*>(maxd2), flag, nbuild, force_all);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_reduce(int R, const int* rep_tiles, const double* partial, int col0, int col1,
+                          double* thermo, void* stream) {
+    if (R <= 0 || !rep_tiles || !partial || !thermo || col0 < 0 || col1 > SIM_NPART ||
+        col0 >= col1)
+        return SIM_E_ARG;
+    hipLaunchKernelGGL(k_reduce, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles, partial, col0,
+                       col1, thermo);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double* thermo,
+                             double* pos4, float* vel, double* pos_last, void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !thermo || !pos4 || !vel || !pos_last)
+        return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_baro_box, dim3((R + 63) / 64), dim3(64), 0, st, R,
+                       static_cast<sim_replica_t*>(reps), thermo);
+    hipLaunchKernelGGL(k_baro_scale, dim3(ntiles), dim3(SIM_TILE), 0, st,
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles), reinterpret_cast<double4*>(pos4), vel,
+                       reinterpret_cast<double4*>(pos_last));
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_pos_export(const double* pos4, int n, void* xyz, int dtype, void* stream) {
+    if (!pos4 || n <= 0 || !xyz) return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    auto* X = reinterpret_cast<const double4*>(pos4);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_export<float>, dim3((n + 255) / 256), dim3(256), 0, st, X, n,
+                           static_cast<float*>(xyz));
+    else if (dtype == 1)
+        hipLaunchKernelGGL(k_export<double>, dim3((n + 255) / 256), dim3(256), 0, st, X, n,
+                           static_cast<double*>(xyz));
+    else
+        return SIM_E_ARG;
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_pos_import(double* pos4, int n, const void* xyz, int dtype, void* stream) {
+    if (!pos4 || n <= 0 || !xyz) return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    auto* X = reinterpret_cast<double4*>(pos4);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_import<float>, dim3((n + 255) / 256), dim3(256), 0, st, X, n,
+                           static_cast<const float*>(xyz));
+    else if (dtype == 1)
+        hipLaunchKernelGGL(k_import<double>, dim3((n + 255) / 256), dim3(256), 0, st, X, n,
+                           static_cast<const double*>(xyz));
+    else
+        return SIM_E_ARG;
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}

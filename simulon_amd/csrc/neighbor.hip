@@ -1,0 +1,375 @@
+// Cell-list Verlet neighbor build for packed replicas (gfx950).
+//
+// Replaces the O(N^2) brute force of core/neighbor_search/gpu_kdtree.py:90-156
+// and the cubic-only, 500-capped kernel of
+// "cuda source/neighbor_search_kernel.cu:6-138". The pair test is the
+// reference's own (minimum image of core/box.py:122-130, squared distance
+// (x*x + y*y) + z*z, strict d^2 < r_l^2, gpu_kdtree.py:132-138) evaluated in
+// float64 for the MD path, or in float32/float64 exactly as the reference's
+// dtype for the find_neighbors API; cells only enumerate candidates, so the
+// pair set equals the brute force bit for bit.
+//
+// Steps (each kernel skips replicas whose rebuild flag is 0):
+//   grid     per replica: cells of perpendicular width >= r_l in fractional
+//            coordinates (triclinic-safe), capped by the replica's capacity
+//   bin      cell id of each atom from its wrapped fractional coordinate,
+//            slot within the cell by an integer atomic counter
+//   scan     exclusive prefix over all cells (one workgroup)
+//   scatter  atom -> cell_atoms[start + slot]
+//   order    per-cell insertion sort by atom index (deterministic lists)
+//   list     thread per atom over the 27 (or fewer, when a dimension has < 3
+//            cells) distinct stencil cells; column-major full list
+#include "sim_common.h"
+
+namespace {
+
+__device__ __forceinline__ bool rep_active(const int* flag, int r) {
+    return flag == nullptr || flag[r] != 0;
+}
+
+__global__ void k_grid(int R, sim_replica_t* reps, const int* flag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R || !rep_active(flag, r)) return;
+    sim_replica_t& P = reps[r];
+    const double* h = P.h;
+    // perpendicular widths: V / |b x c|, V / |c x a|, V / |a x b|
+    double a[3] = {h[0], h[1], h[2]}, b[3] = {h[3], h[4], h[5]}, c[3] = {h[6], h[7], h[8]};
+    auto cross_norm = [](const double* u, const double* v) {
+        double x = u[1] * v[2] - u[2] * v[1];
+        double y = u[2] * v[0] - u[0] * v[2];
+        double z = u[0] * v[1] - u[1] * v[0];
+        return sqrt(x * x + y * y + z * z);
+    };
+    double V = fabs(a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                    a[2] * (b[0] * c[1] - b[1] * c[0]));
+    double w[3] = {V / cross_norm(b, c), V / cross_norm(c, a), V / cross_norm(a, b)};
+    int nc[3];
+    for (int k = 0; k < 3; ++k) {
+        double q = floor(w[k] / (P.rl * (1.0 + 1e-6)));
+        nc[k] = q < 1.0 ? 1 : (q > 1024.0 ? 1024 : (int)q);
+    }
+    // shrink the largest dimension until the grid fits the reserved capacity
+    while ((long long)nc[0] * nc[1] * nc[2] > P.cell_cap) {
+        int m = (nc[0] >= nc[1] && nc[0] >= nc[2]) ? 0 : (nc[1] >= nc[2] ? 1 : 2);
+        nc[m] -= 1;
+    }
+    P.ncell[0] = nc[0];
+    P.ncell[1] = nc[1];
+    P.ncell[2] = nc[2];
+    P.mu_acc = 1.0;
+}
+
+__global__ void k_cell_clear(const int* cell_rep, int ncells, const int* flag, int* cell_count) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    if (rep_active(flag, cell_rep ? cell_rep[c] : 0)) cell_count[c] = 0;
+}
+
+template <typename T, int STRIDE>
+__device__ __forceinline__ void load_xyz(const T* x, int i, double& a, double& b, double& c) {
+    a = (double)x[(size_t)i * STRIDE + 0];
+    b = (double)x[(size_t)i * STRIDE + 1];
+    c = (double)x[(size_t)i * STRIDE + 2];
+}
+
+__device__ __forceinline__ int cell_of(const sim_replica_t& P, double x, double y, double z) {
+    const double* hi = P.hinv;
+    double s[3];
+    s[0] = x * hi[0] + y * hi[3] + z * hi[6];
+    s[1] = x * hi[1] + y * hi[4] + z * hi[7];
+    s[2] = x * hi[2] + y * hi[5] + z * hi[8];
+    int c[3];
+    for (int k = 0; k < 3; ++k) {
+        double f = s[k] - floor(s[k]);
+        int q = (int)(f * P.ncell[k]);
+        c[k] = q < 0 ? 0 : (q >= P.ncell[k] ? P.ncell[k] - 1 : q);
+    }
+    return P.cell_base + (c[2] * P.ncell[1] + c[1]) * P.ncell[0] + c[0];
+}
+
+template <typename T, int STRIDE>
+__global__ void k_bin(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
+                      int* cell_count, int* atom_cell, int* atom_slot) {
+    const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    double a, b, c;
+    load_xyz<T, STRIDE>(x, i, a, b, c);
+    int cell = cell_of(P, a, b, c);
+    atom_cell[i] = cell;
+    atom_slot[i] = atomicAdd(&cell_count[cell], 1);
+}
+
+// Exclusive scan of cell_count[0..n) into cell_start[0..n], one workgroup of
+// 1024 threads, each owning a contiguous chunk.
+__global__ void __launch_bounds__(1024) k_scan(const int* cnt, int n, int* start) {
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int b = tid * chunk, e = min(n, b + chunk);
+    int s = 0;
+    for (int k = b; k < e; ++k) s += cnt[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - s;
+    for (int k = b; k < e; ++k) {
+        start[k] = run;
+        run += cnt[k];
+    }
+    if (tid == 1023) start[n] = part[1023];
+}
+
+__global__ void k_scatter(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
+                          const int* atom_cell, const int* atom_slot, const int* cell_start,
+                          int* cell_atoms) {
+    const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    cell_atoms[cell_start[atom_cell[i]] + atom_slot[i]] = i;
+}
+
+__global__ void k_cell_order(const int* cell_rep, int ncells, const int* flag,
+                             const int* cell_count, const int* cell_start, int* cell_atoms) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells || !rep_active(flag, cell_rep ? cell_rep[c] : 0)) return;
+    int* a = cell_atoms + cell_start[c];
+    const int n = cell_count[c];
+    for (int k = 1; k < n; ++k) {
+        int v = a[k];
+        int m = k - 1;
+        while (m >= 0 && a[m] > v) {
+            a[m + 1] = a[m];
+            --m;
+        }
+        a[m + 1] = v;
+    }
+}
+
+// Stencil offsets of one dimension: distinct cells only.
+__device__ __forceinline__ int stencil(int nc, int* off) {
+    if (nc == 1) {
+        off[0] = 0;
+        return 1;
+    }
+    if (nc == 2) {
+        off[0] = 0;
+        off[1] = 1;
+        return 2;
+    }
+    off[0] = -1;
+    off[1] = 0;
+    off[2] = 1;
+    return 3;
+}
+
+template <typename TC, typename T, int STRIDE>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_list(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
+           const int* atom_cell, const int* cell_count, const int* cell_start,
+           const int* cell_atoms, int* nbr, int* nnbr, int n_stride, int max_nbr,
+           int* overflow, double* pos_last) {
+    const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const TC xi = (TC)x[(size_t)i * STRIDE + 0];
+    const TC yi = (TC)x[(size_t)i * STRIDE + 1];
+    const TC zi = (TC)x[(size_t)i * STRIDE + 2];
+    const TC rl2 = (TC)P.rl2;
+    const int nx = P.ncell[0], ny = P.ncell[1], nz = P.ncell[2];
+    int cl = atom_cell[i] - P.cell_base;
+    const int cx = cl % nx, cy = (cl / nx) % ny, cz = cl / (nx * ny);
+    int ox[3], oy[3], oz[3];
+    const int sx = stencil(nx, ox), sy = stencil(ny, oy), sz = stencil(nz, oz);
+    int cnt = 0;
+    for (int a = 0; a < sz; ++a) {
+        int zz = cz + oz[a];
+        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        for (int b = 0; b < sy; ++b) {
+            int yy = cy + oy[b];
+            yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+            for (int c = 0; c < sx; ++c) {
+                int xx = cx + ox[c];
+                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                const int cell = P.cell_base + (zz * ny + yy) * nx + xx;
+                const int s0 = cell_start[cell], s1 = s0 + cell_count[cell];
+                for (int s = s0; s < s1; ++s) {
+                    const int j = cell_atoms[s];
+                    if (j == i) continue;
+                    TC dx = xi - (TC)x[(size_t)j * STRIDE + 0];
+                    TC dy = yi - (TC)x[(size_t)j * STRIDE + 1];
+                    TC dz = zi - (TC)x[(size_t)j * STRIDE + 2];
+                    min_image_ref<TC>(dx, dy, dz, P);
+                    const TC d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (d2 < rl2) {
+                        if (cnt < max_nbr) nbr[(size_t)cnt * n_stride + i] = j;
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    nnbr[i] = cnt < max_nbr ? cnt : max_nbr;
+    if (cnt > max_nbr) atomicOr(overflow, 1);
+    if (pos_last != nullptr) {
+        pos_last[(size_t)i * 4 + 0] = (double)x[(size_t)i * STRIDE + 0];
+        pos_last[(size_t)i * 4 + 1] = (double)x[(size_t)i * STRIDE + 1];
+        pos_last[(size_t)i * 4 + 2] = (double)x[(size_t)i * STRIDE + 2];
+        pos_last[(size_t)i * 4 + 3] = 0.0;
+    }
+}
+
+template <typename TC, typename T, int STRIDE>
+int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, const T* x,
+               double* pos_last, const int* flag, const int* cell_rep, int ncells, int* atom_cell,
+               int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms, int* nbr,
+               int* nnbr, int n_stride, int max_nbr, int* overflow, hipStream_t st) {
+    hipLaunchKernelGGL(k_grid, dim3((R + 63) / 64), dim3(64), 0, st, R, reps, flag);
+    hipLaunchKernelGGL(k_cell_clear, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
+                       ncells, flag, cell_count);
+    hipLaunchKernelGGL((k_bin<T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, x,
+                       flag, cell_count, atom_cell, atom_slot);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, ncells, cell_start);
+    hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                       atom_cell, atom_slot, cell_start, cell_atoms);
+    hipLaunchKernelGGL(k_cell_order, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
+                       ncells, flag, cell_count, cell_start, cell_atoms);
+    hipLaunchKernelGGL((k_list<TC, T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles,
+                       x, flag, atom_cell, cell_count, cell_start, cell_atoms, nbr, nnbr, n_stride,
+                       max_nbr, overflow, pos_last);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+// ---- half list for the find_neighbors API ---------------------------------
+
+__global__ void k_half_count(const int* nbr, const int* nnbr, int n, int* hc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    for (int k = 0; k < nnbr[i]; ++k) c += nbr[(size_t)k * n + i] > i;
+    hc[i] = c;
+}
+
+template <typename T>
+__global__ void k_half_fill(const sim_replica_t* rep, const T* x, int n, const int* nbr,
+                            const int* nnbr, const int* hs, long long* ei, long long E, T* dist) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int base = hs[i];
+    int m = 0;
+    // selection by insertion into the output slice, ascending j
+    for (int k = 0; k < nnbr[i]; ++k) {
+        const int j = nbr[(size_t)k * n + i];
+        if (j <= i) continue;
+        int p = m;
+        while (p > 0 && ei[E + base + p - 1] > j) {
+            ei[E + base + p] = ei[E + base + p - 1];
+            --p;
+        }
+        ei[E + base + p] = j;
+        ++m;
+    }
+    const sim_replica_t& P = *rep;
+    for (int k = 0; k < m; ++k) {
+        const int j = (int)ei[E + base + k];
+        ei[base + k] = i;
+        T dx = x[(size_t)i * 3 + 0] - x[(size_t)j * 3 + 0];
+        T dy = x[(size_t)i * 3 + 1] - x[(size_t)j * 3 + 1];
+        T dz = x[(size_t)i * 3 + 2] - x[(size_t)j * 3 + 2];
+        min_image_ref<T>(dx, dy, dz, P);
+        dist[base + k] = sqrt((dx * dx + dy * dy) + dz * dz);
+    }
+}
+
+}  // namespace
+
+extern "C" int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
+                            const double* pos4, double* pos_last, const int* flag,
+                            const int* cell_rep, int ncells, int* atom_cell, int* atom_slot,
+                            int* cell_count, int* cell_start, int* cell_atoms, int* nbr, int* nnbr,
+                            int max_nbr, int* overflow, int precise, void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || n <= 0 || !pos4 || !cell_rep || ncells <= 0 ||
+        !atom_cell || !atom_slot || !cell_count || !cell_start || !cell_atoms || !nbr || !nnbr ||
+        max_nbr <= 0 || !overflow)
+        return SIM_E_ARG;
+    auto* P = static_cast<sim_replica_t*>(reps);
+    auto* T = reinterpret_cast<const SimTile*>(tiles);
+    if (precise)
+        return build_impl<double, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
+                                             atom_cell, atom_slot, cell_count, cell_start,
+                                             cell_atoms, nbr, nnbr, n, max_nbr, overflow,
+                                             as_stream(stream));
+    return build_impl<float, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
+                                        atom_cell, atom_slot, cell_count, cell_start, cell_atoms,
+                                        nbr, nnbr, n, max_nbr, overflow, as_stream(stream));
+}
+
+extern "C" int sim_nl_build_xyz(void* rep, const void* xyz, int dtype, int n, int* atom_cell,
+                                int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms,
+                                int ncells, int* nbr, int* nnbr, int max_nbr, int* overflow,
+                                void* stream) {
+    if (!rep || !xyz || n <= 0 || !atom_cell || !atom_slot || !cell_count || !cell_start ||
+        !cell_atoms || ncells <= 0 || !nbr || !nnbr || max_nbr <= 0 || !overflow)
+        return SIM_E_ARG;
+    auto* P = static_cast<sim_replica_t*>(rep);
+    const int ntiles = (n + SIM_TILE - 1) / SIM_TILE;
+    hipStream_t st = as_stream(stream);
+    if (dtype == 0)
+        return build_impl<float, float, 3>(1, P, nullptr, ntiles, static_cast<const float*>(xyz),
+                                           nullptr, nullptr, nullptr, ncells, atom_cell, atom_slot,
+                                           cell_count, cell_start, cell_atoms, nbr, nnbr, n,
+                                           max_nbr, overflow, st);
+    if (dtype == 1)
+        return build_impl<double, double, 3>(1, P, nullptr, ntiles,
+                                             static_cast<const double*>(xyz), nullptr, nullptr,
+                                             nullptr, ncells, atom_cell, atom_slot, cell_count,
+                                             cell_start, cell_atoms, nbr, nnbr, n, max_nbr,
+                                             overflow, st);
+    return SIM_E_ARG;
+}
+
+extern "C" int sim_nl_half_count(const int* nbr, const int* nnbr, int n, int max_nbr,
+                                 int* half_count, int* half_start, void* stream) {
+    if (!nbr || !nnbr || n <= 0 || max_nbr <= 0 || !half_count || !half_start) return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_half_count, dim3((n + 255) / 256), dim3(256), 0, st, nbr, nnbr, n,
+                       half_count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, half_count, n, half_start);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_nl_half_fill(void* rep, const void* xyz, int dtype, int n, const int* nbr,
+                                const int* nnbr, int max_nbr, const int* half_start,
+                                long long* edge_index, long long E, void* dist, void* stream) {
+    if (!rep || !xyz || n <= 0 || !nbr || !nnbr || max_nbr <= 0 || !half_start || E < 0)
+        return SIM_E_ARG;
+    if (E == 0) return SIM_OK;
+    if (!edge_index || !dist) return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    auto* P = static_cast<const sim_replica_t*>(rep);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_half_fill<float>, dim3((n + 255) / 256), dim3(256), 0, st, P,
+                           static_cast<const float*>(xyz), n, nbr, nnbr, half_start, edge_index,
+                           E, static_cast<float*>(dist));
+    else if (dtype == 1)
+        hipLaunchKernelGGL(k_half_fill<double>, dim3((n + 255) / 256), dim3(256), 0, st, P,
+                           static_cast<const double*>(xyz), n, nbr, nnbr, half_start, edge_index,
+                           E, static_cast<double*>(dist));
+    else
+        return SIM_E_ARG;
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}

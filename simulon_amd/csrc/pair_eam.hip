@@ -1,0 +1,230 @@
+// Two-pass EAM over the full Verlet list (gfx950).
+//
+// Physics of core/force/eam_force_cu.py:461-579 (tables from :334-395, clamped
+// linear interpolation :437-459):
+//   pass 1  rho_i = sum_j f_{ti,tj}(r_ij)            (r < cutoff)
+//           E_i   = F_ti(rho_i) + 1/2 sum_j phi_{ti,tj}(r_ij);  fp_i = F'_ti(rho_i)
+//   pass 2  s_ij  = fp_i f'_{ti,tj}(r) + fp_j f'_{tj,ti}(r) + phi'_{ti,tj}(r)
+//           F_i   = sum_j -s_ij rij / (r + 1e-8)
+//           virial = 1/2 sum s r ; W = 1/2 sum (-s/(r+1e-8)) rij (x) rij
+// The reference computes rho with 2 float atomics per edge and forces with 6
+// ("cuda source/eam_cuda_ext_kernel.cu:47-52, 108-113"); here each thread owns
+// one atom of the full list, so both passes are register sums.
+//
+// Tables are interleaved so that one 16-byte load yields both grid values a
+// linear interpolation needs for two functions: pass1[k] = (f(k), f(k+1),
+// phi(k), phi(k+1)), pass2[k] = (f'(k), f'(k+1), phi'(k), phi'(k+1)), with
+// the k+1 entry clamped at the last grid point as the reference clamps `next`.
+#include "sim_common.h"
+
+namespace {
+
+struct Lerp {
+    int idx;
+    float frac;
+};
+
+__device__ __forceinline__ Lerp lerp_r(float r, const sim_eam_params_t& E) {
+    const float rc = fminf(fmaxf(r, 0.0f), E.r_clamp);
+    const float xf = rc * E.inv_dr;
+    const int idx = (int)xf;
+    return Lerp{idx, xf - (float)idx};
+}
+
+__device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t * (b - a); }
+
+template <int NT>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+                  const double4* __restrict__ pos, const int* __restrict__ nbr,
+                  const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
+                  float* __restrict__ rho_out, float* __restrict__ fp_out,
+                  double* __restrict__ partial, float rc2_thr) {
+    __shared__ double lds[4 * 1];
+    const SimTile t = tiles[blockIdx.x];
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    const bool active = i < P.atom_base + P.natoms;
+    const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
+    double etot = 0.0;
+    if (active) {
+        const double4 xi = pos[i];
+        const int ti = (int)xi.w;
+        const int cnt = nnbr[i];
+        float rho = 0.f, ep = 0.f;
+        for (int k = 0; k < cnt; ++k) {
+            const int j = nbr[(size_t)k * n_stride + i];
+            const double4 xj = pos[j];
+            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            min_image_fast(dx, dy, dz, P);
+            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
+            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+            if (r2 < rc2_thr) {
+                const int tj = NT == 1 ? 0 : (int)xj.w;
+                const int nt = NT == 1 ? 1 : E.ntypes;
+                const Lerp L = lerp_r(sqrtf(r2), E);
+                const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
+                rho += lerp2(v.x, v.y, L.frac);
+                ep += lerp2(v.z, v.w, L.frac);
+            }
+        }
+        // embedding, eam_force_cu.py:448-459
+        const float lo = E.rho_lo[ti];
+        const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
+        const float xf = (rcl - lo) * E.inv_drho[ti];
+        const int idx = (int)xf;
+        const float fr = xf - (float)idx;
+        const float2 F = reinterpret_cast<const float2*>(E.emb)[(size_t)ti * E.n_rho + idx];
+        const float2 dF = reinterpret_cast<const float2*>(E.demb)[(size_t)ti * E.n_rho + idx];
+        rho_out[i] = rho;
+        fp_out[i] = lerp2(dF.x, dF.y, fr);
+        etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
+    }
+    double v[1] = {etot};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * SIM_NPART + SIM_T_EPOT] = v[0];
+}
+
+template <int NT, bool VIRIAL>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+                const double4* __restrict__ pos, const int* __restrict__ nbr,
+                const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
+                const float* __restrict__ fp, float* __restrict__ force,
+                double* __restrict__ partial, float rc2_thr) {
+    __shared__ double lds[4 * 7];
+    const SimTile t = tiles[blockIdx.x];
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    const bool active = i < P.atom_base + P.natoms;
+    const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
+    float fx = 0.f, fy = 0.f, fz = 0.f, vir = 0.f;
+    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    if (active) {
+        const double4 xi = pos[i];
+        const int ti = (int)xi.w;
+        const int cnt = nnbr[i];
+        const float fpi = fp[i];
+        for (int k = 0; k < cnt; ++k) {
+            const int j = nbr[(size_t)k * n_stride + i];
+            const double4 xj = pos[j];
+            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            min_image_fast(dx, dy, dz, P);
+            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
+            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+            if (r2 < rc2_thr) {
+                const float r = sqrtf(r2);
+                const Lerp L = lerp_r(r, E);
+                const float fpj = fp[j];
+                float s;
+                if (NT == 1) {
+                    const float4 v = T2[L.idx];
+                    const float dd = lerp2(v.x, v.y, L.frac);
+                    s = fpi * dd + fpj * dd + lerp2(v.z, v.w, L.frac);
+                } else {
+                    const int tj = (int)xj.w;
+                    const int nt = E.ntypes;
+                    const float4 a = T2[(size_t)(ti * nt + tj) * E.n_r + L.idx];
+                    const float4 b = T2[(size_t)(tj * nt + ti) * E.n_r + L.idx];
+                    s = fpi * lerp2(a.x, a.y, L.frac) + fpj * lerp2(b.x, b.y, L.frac) +
+                        lerp2(a.z, a.w, L.frac);
+                }
+                const float g = -s / (r + 1e-8f);
+                fx = __builtin_fmaf(g, rx, fx);
+                fy = __builtin_fmaf(g, ry, fy);
+                fz = __builtin_fmaf(g, rz, fz);
+                if (VIRIAL) {
+                    vir = __builtin_fmaf(s, r, vir);
+                    const float gx = g * rx, gy = g * ry;
+                    wxx = __builtin_fmaf(gx, rx, wxx);
+                    wyy = __builtin_fmaf(gy, ry, wyy);
+                    wzz = __builtin_fmaf(g * rz, rz, wzz);
+                    wxy = __builtin_fmaf(gx, ry, wxy);
+                    wxz = __builtin_fmaf(gx, rz, wxz);
+                    wyz = __builtin_fmaf(gy, rz, wyz);
+                }
+            }
+        }
+        force[(size_t)i * 3 + 0] = fx;
+        force[(size_t)i * 3 + 1] = fy;
+        force[(size_t)i * 3 + 2] = fz;
+    }
+    double v[7] = {0.5 * vir, 0.5 * wxx, 0.5 * wyy, 0.5 * wzz, 0.5 * wxy, 0.5 * wxz, 0.5 * wyz};
+    block_sum<7>(v, lds);
+    if (threadIdx.x == 0) {
+        double* p = partial + (size_t)blockIdx.x * SIM_NPART;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
+    }
+}
+
+float sqrt_threshold(float rc) {
+    float x = rc * rc;
+    while (sqrtf(x) < rc) x = nextafterf(x, 1e30f);
+    while (true) {
+        float y = nextafterf(x, 0.f);
+        if (sqrtf(y) >= rc) x = y;
+        else break;
+    }
+    return x;
+}
+
+}  // namespace
+
+extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
+                               const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                               sim_eam_params_t eam, float* rho, float* fp, double* partial,
+                               void* stream) {
+    (void)R;
+    if (!reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 || !rho ||
+        !fp || !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
+        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES)
+        return SIM_E_ARG;
+    const float thr = sqrt_threshold(eam.cutoff);
+    auto* P = static_cast<const sim_replica_t*>(reps);
+    auto* T = reinterpret_cast<const SimTile*>(tiles);
+    auto* X = reinterpret_cast<const double4*>(pos4);
+    hipStream_t st = as_stream(stream);
+    if (eam.ntypes == 1)
+        hipLaunchKernelGGL((k_eam_density<1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
+                           nnbr, n_stride, eam, rho, fp, partial, thr);
+    else
+        hipLaunchKernelGGL((k_eam_density<0>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
+                           nnbr, n_stride, eam, rho, fp, partial, thr);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
+                             const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                             sim_eam_params_t eam, const float* fp, float* force, double* partial,
+                             int virial, void* stream) {
+    (void)R;
+    if (!reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 || !fp ||
+        !force || !partial || !eam.pass2 || eam.n_r < 2 || eam.ntypes < 1 ||
+        eam.ntypes > SIM_MAX_TYPES)
+        return SIM_E_ARG;
+    const float thr = sqrt_threshold(eam.cutoff);
+    auto* P = static_cast<const sim_replica_t*>(reps);
+    auto* T = reinterpret_cast<const SimTile*>(tiles);
+    auto* X = reinterpret_cast<const double4*>(pos4);
+    hipStream_t st = as_stream(stream);
+    const dim3 g(ntiles), b(SIM_TILE);
+    if (eam.ntypes == 1) {
+        if (virial)
+            hipLaunchKernelGGL((k_eam_force<1, true>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
+                               eam, fp, force, partial, thr);
+        else
+            hipLaunchKernelGGL((k_eam_force<1, false>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
+                               eam, fp, force, partial, thr);
+    } else {
+        if (virial)
+            hipLaunchKernelGGL((k_eam_force<0, true>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
+                               eam, fp, force, partial, thr);
+        else
+            hipLaunchKernelGGL((k_eam_force<0, false>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
+                               eam, fp, force, partial, thr);
+    }
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}

@@ -1,0 +1,163 @@
+// Lennard-Jones energy / force / virial over the full Verlet list (gfx950).
+//
+// Physics of core/force/lennard_jones_force.py:15-95 (raw, force-shift and
+// polynomial-switch forms; only r < rc counts), restructured for the GPU:
+//   * one thread per atom walks its own column of the column-major full list
+//     (coalesced index loads), so the per-atom force is a register sum — no
+//     atomics, deterministic order (the reference's kernel issues 6 float
+//     atomics per edge, "cuda source/lj_energy_force_kernel.cu:109-116");
+//   * the displacement is formed in float64 from float64 positions and
+//     rounded once to float32; everything after that is float32;
+//   * energy and virial are halved per atom (each pair is seen from both
+//     sides), reduced over the workgroup in float64 with wave shuffles, one
+//     partial per tile; sim_reduce adds the tiles with compensation.
+#include "sim_common.h"
+
+namespace {
+
+template <bool VIRIAL, int MODE>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+         const double4* __restrict__ pos, const int* __restrict__ nbr,
+         const int* __restrict__ nnbr, int n_stride, sim_lj_params_t lj,
+         float* __restrict__ force, double* __restrict__ partial, float rc2_thr) {
+    __shared__ double lds[4 * 8];
+    const SimTile t = tiles[blockIdx.x];
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    const bool active = i < P.atom_base + P.natoms;
+    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, vir = 0.f;
+    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    if (active) {
+        const double4 xi = pos[i];
+        const int ti = (int)xi.w;
+        const int cnt = nnbr[i];
+        const float rc = lj.rc;
+        for (int k = 0; k < cnt; ++k) {
+            const int j = nbr[(size_t)k * n_stride + i];
+            const double4 xj = pos[j];
+            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            min_image_fast(dx, dy, dz, P);
+            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
+            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+            if (r2 < rc2_thr) {
+                const int tj = (int)xj.w;
+                const float eps = lj.eps[ti * SIM_MAX_TYPES + tj];
+                const float sig = lj.sig[ti * SIM_MAX_TYPES + tj];
+                const float inv_r2 = 1.0f / r2;
+                const float s2 = sig * sig * inv_r2;
+                const float s6 = s2 * s2 * s2;
+                const float s12 = s6 * s6;
+                float U = 4.0f * eps * (s12 - s6);
+                float fr = 24.0f * eps * inv_r2 * (2.0f * s12 - s6);  // Fmag / r
+                if (MODE == 1) {  // force shift, lennard_jones_force.py:65-75
+                    const float r = sqrtf(r2);
+                    const float src = sig / rc;
+                    const float q6 = src * src * src * src * src * src;
+                    const float q12 = q6 * q6;
+                    const float Urc = 4.0f * eps * (q12 - q6);
+                    const float Frc = 24.0f * eps * (1.0f / rc) * (2.0f * q12 - q6);
+                    U = U - Urc + (r - rc) * Frc;
+                    fr = fr - Frc / r;
+                } else if (MODE == 2) {  // switch, lennard_jones_force.py:48-63
+                    const float r = sqrtf(r2);
+                    const float rs = lj.rs;
+                    if (r > rs) {
+                        float x = (r - rs) / (rc - rs);
+                        x = fminf(fmaxf(x, 0.f), 1.f);
+                        const float x2 = x * x, x3 = x2 * x;
+                        const float S = 1.0f - 10.0f * x3 + 15.0f * x3 * x - 6.0f * x3 * x2;
+                        const float dS = (-30.0f * x2 + 60.0f * x3 - 30.0f * x2 * x2) / (rc - rs);
+                        fr = fr * S - U * dS / r;
+                        U = U * S;
+                    }
+                }
+                fx = __builtin_fmaf(fr, rx, fx);
+                fy = __builtin_fmaf(fr, ry, fy);
+                fz = __builtin_fmaf(fr, rz, fz);
+                e += U;
+                if (VIRIAL) {
+                    vir = __builtin_fmaf(fr, r2, vir);
+                    const float gx = fr * rx, gy = fr * ry;
+                    wxx = __builtin_fmaf(gx, rx, wxx);
+                    wyy = __builtin_fmaf(gy, ry, wyy);
+                    wzz = __builtin_fmaf(fr * rz, rz, wzz);
+                    wxy = __builtin_fmaf(gx, ry, wxy);
+                    wxz = __builtin_fmaf(gx, rz, wxz);
+                    wyz = __builtin_fmaf(gy, rz, wyz);
+                }
+            }
+        }
+        force[(size_t)i * 3 + 0] = fx;
+        force[(size_t)i * 3 + 1] = fy;
+        force[(size_t)i * 3 + 2] = fz;
+    }
+    double v[8] = {0.5 * e, 0.5 * vir, 0.5 * wxx, 0.5 * wyy, 0.5 * wzz,
+                   0.5 * wxy, 0.5 * wxz, 0.5 * wyz};
+    block_sum<8>(v, lds);
+    if (threadIdx.x == 0) {
+        double* p = partial + (size_t)blockIdx.x * SIM_NPART;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p[k] = v[k];
+    }
+}
+
+// Smallest float x with sqrtf(x) >= rc: r2 < thr  <=>  sqrtf(r2) < rc, so the
+// squared test reproduces the reference's `r < rc` mask exactly.
+float sqrt_threshold(float rc) {
+    float x = rc * rc;
+    while (sqrtf(x) < rc) x = nextafterf(x, 1e30f);
+    while (true) {
+        float y = nextafterf(x, 0.f);
+        if (sqrtf(y) >= rc) x = y;
+        else break;
+    }
+    return x;
+}
+
+template <bool V>
+void launch_mode(int mode, dim3 g, hipStream_t st, const sim_replica_t* reps, const SimTile* tiles,
+                 const double4* pos, const int* nbr, const int* nnbr, int ns, sim_lj_params_t lj,
+                 float* f, double* part, float thr) {
+    if (mode == 1)
+        hipLaunchKernelGGL((k_lj<V, 1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
+    else if (mode == 2)
+        hipLaunchKernelGGL((k_lj<V, 2>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
+    else
+        hipLaunchKernelGGL((k_lj<V, 0>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
+}
+
+}  // namespace
+
+extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
+                            const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                            sim_lj_params_t lj, float* force, double* partial, int virial,
+                            void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 ||
+        !force || !partial || lj.mode < 0 || lj.mode > 2 || !(lj.rc > 0.f))
+        return SIM_E_ARG;
+    const float thr = sqrt_threshold(lj.rc);
+    auto* P = static_cast<const sim_replica_t*>(reps);
+    auto* T = reinterpret_cast<const SimTile*>(tiles);
+    auto* X = reinterpret_cast<const double4*>(pos4);
+    if (virial)
+        launch_mode<true>(lj.mode, dim3(ntiles), as_stream(stream), P, T, X, nbr, nnbr, n_stride, lj,
+                          force, partial, thr);
+    else
+        launch_mode<false>(lj.mode, dim3(ntiles), as_stream(stream), P, T, X, nbr, nnbr, n_stride,
+                           lj, force, partial, thr);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_version(char* buf, int len) {
+    const char* v = "simulon_hip 0.1 gfx950";
+    if (!buf || len <= 0) return SIM_E_ARG;
+    int k = 0;
+    for (; k < len - 1 && v[k]; ++k) buf[k] = v[k];
+    buf[k] = 0;
+    return SIM_OK;
+}

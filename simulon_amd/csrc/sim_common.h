@@ -1,0 +1,139 @@
+// Device helpers shared by the MD-step kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/simulon_hip.h"
+
+#define SIM_CHECK_LAUNCH()                                   \
+    do {                                                     \
+        if (hipGetLastError() != hipSuccess) return SIM_E_LAUNCH; \
+    } while (0)
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Tile table entry: (replica, first global atom index).
+struct SimTile {
+    int rep;
+    int start;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum of NV doubles per thread over a 256-thread workgroup; thread 0 of the
+// workgroup receives the totals in v[]. Tree order is fixed (deterministic).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [4*NV] */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[wid * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            v[k] = (lds[k] + lds[NV + k]) + (lds[2 * NV + k] + lds[3 * NV + k]);
+    }
+}
+
+// Minimum image of a displacement with the reference's arithmetic
+// (core/box.py:122-130), in T = float or double, contraction-free (the
+// library is compiled with -ffp-contract=off).
+template <typename T>
+__device__ __forceinline__ void min_image_ref(T& dx, T& dy, T& dz, const sim_replica_t& R) {
+    if (R.ortho) {
+        const T lx = (T)R.h[0], ly = (T)R.h[4], lz = (T)R.h[8];
+        dx = dx - lx * rint(dx / lx);
+        dy = dy - ly * rint(dy / ly);
+        dz = dz - lz * rint(dz / lz);
+    } else {
+        const double* hi = R.hinv;
+        const double* h = R.h;
+        T s0 = (dx * (T)hi[0] + dy * (T)hi[3]) + dz * (T)hi[6];
+        T s1 = (dx * (T)hi[1] + dy * (T)hi[4]) + dz * (T)hi[7];
+        T s2 = (dx * (T)hi[2] + dy * (T)hi[5]) + dz * (T)hi[8];
+        s0 = s0 - rint(s0);
+        s1 = s1 - rint(s1);
+        s2 = s2 - rint(s2);
+        dx = (s0 * (T)h[0] + s1 * (T)h[3]) + s2 * (T)h[6];
+        dy = (s0 * (T)h[1] + s1 * (T)h[4]) + s2 * (T)h[7];
+        dz = (s0 * (T)h[2] + s1 * (T)h[5]) + s2 * (T)h[8];
+    }
+}
+
+// Minimum image for force evaluation in float64: multiplies by 1/L (and by
+// H^-1) instead of dividing; any image choice that differs from the
+// reference's can only happen at an exact half-box separation, far beyond
+// the cutoff.
+__device__ __forceinline__ void min_image_fast(double& dx, double& dy, double& dz,
+                                               const sim_replica_t& R) {
+    if (R.ortho) {
+        dx = fma(-R.h[0], rint(dx * R.hinv[0]), dx);
+        dy = fma(-R.h[4], rint(dy * R.hinv[4]), dy);
+        dz = fma(-R.h[8], rint(dz * R.hinv[8]), dz);
+    } else {
+        const double* hi = R.hinv;
+        const double* h = R.h;
+        double s0 = fma(dz, hi[6], fma(dy, hi[3], dx * hi[0]));
+        double s1 = fma(dz, hi[7], fma(dy, hi[4], dx * hi[1]));
+        double s2 = fma(dz, hi[8], fma(dy, hi[5], dx * hi[2]));
+        s0 = rint(s0);
+        s1 = rint(s1);
+        s2 = rint(s2);
+        dx = fma(-s2, h[6], fma(-s1, h[3], fma(-s0, h[0], dx)));
+        dy = fma(-s2, h[7], fma(-s1, h[4], fma(-s0, h[1], dy)));
+        dz = fma(-s2, h[8], fma(-s1, h[5], fma(-s0, h[2], dz)));
+    }
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) --------------------
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Uniform in (0, 1] from 32 random bits.
+__device__ __forceinline__ float u01(uint32_t b) {
+    return ((float)(b >> 8) + 1.0f) * (1.0f / 16777216.0f);
+}
+
+// Three N(0,1) draws for (atom, step) by Box-Muller on one Philox block.
+__device__ __forceinline__ void gauss3(unsigned long long seed, unsigned long long step,
+                                       uint32_t atom, float& g0, float& g1, float& g2) {
+    u32x4 c{atom, (uint32_t)step, (uint32_t)(step >> 32), 0x5eedu};
+    u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    float ra = sqrtf(-2.0f * logf(u01(r.x)));
+    float rb = sqrtf(-2.0f * logf(u01(r.z)));
+    float sa, ca, sb, cb;
+    sincospif(2.0f * u01(r.y), &sa, &ca);
+    sincospif(2.0f * u01(r.w), &sb, &cb);
+    g0 = ra * ca;
+    g1 = ra * sa;
+    g2 = rb * cb;
+    (void)sb;
+}

@@ -1,0 +1,413 @@
+"""Device-resident MD engine over packed replicas (one GPU).
+
+One `ReplicaEngine` owns the state of R >= 1 independent systems in HBM and
+advances all of them with one launch sequence per step:
+
+    first half (B+A, float64 positions, displacement max)
+    -> skin trigger (per-replica rebuild flag, no host sync)
+    -> cell-list rebuild, gated per replica by the flag
+    -> force field (LJ, or EAM density + force)
+    -> second half (O+B, Philox Langevin kicks, KE / kinetic tensor)
+    -> float64 compensated reductions -> Berendsen (NPT replicas)
+
+The sequence contains no host synchronisation and no allocation, so it is
+captured once into a HIP graph (torch.cuda.CUDAGraph) and replayed. A single
+system is R = 1; the batched replica scan is the same code with R = 64.
+
+Reference behaviour mirrored (JinZongxiao/Simulon):
+  step order                core/md_model.py:88-166
+  skin/2 rebuild trigger    io_utils/reader.py:136-149 (displacement since the
+                            last build; with Berendsen the reference resets its
+                            reference positions every step, core/barostat.py:67,
+                            so it never rebuilds — here the reference positions
+                            are scaled with the box instead and the threshold
+                            accounts for the accumulated scaling)
+  integrator constants      core/integrator/integrator.py:41-87
+  Berendsen                 core/barostat.py:21-68
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_ptr
+
+KB = 8.617333262e-5
+N_A = 6.02214076e23
+J_PER_EV = 1.60218e-19
+MASS_CONV = 10.0 / (N_A * J_PER_EV)
+MB_MASS_CONV = 1.036427e-4
+
+
+def internal_mass_f32(amu):
+    """integrator.py:61-64: float32(amu) * float32(10 / (N_A J_per_eV))."""
+    return np.float32(np.float32(amu) * np.float32(MASS_CONV))
+
+
+def langevin_f32(mass_int, dt, gamma, T):
+    """integrator.py:84-87,110 in float32: (damp, kick std)."""
+    g = np.float32(gamma)
+    dtf = np.float32(dt)
+    exp_term = np.exp(np.float32(-2.0) * g * dtf)
+    rff = np.sqrt((np.float32(KB) * np.float32(T)) / np.float32(mass_int) * (np.float32(1.0) - exp_term))
+    damp = np.exp(-g * dtf)
+    return np.float32(damp), np.float32(rff)
+
+
+def list_radius_sq(cutoff, skin):
+    """reader.py:50,77-78 + gpu_kdtree.py:99: (fp32 cutoff + fp32 skin)^2 in float32."""
+    rl = np.float32(cutoff) + np.float32(skin)
+    return float(np.float32(rl * rl)), float(rl)
+
+
+def perpendicular_widths(H):
+    H = np.asarray(H, np.float64)
+    V = abs(np.linalg.det(H))
+    a, b, c = H
+    return np.array([V / np.linalg.norm(np.cross(b, c)), V / np.linalg.norm(np.cross(c, a)),
+                     V / np.linalg.norm(np.cross(a, b))])
+
+
+@dataclass
+class ReplicaSpec:
+    """One independent system: atoms, box and thermodynamic controls."""
+    positions: np.ndarray                 # [N,3] Angstrom (float64 or float32)
+    box: np.ndarray                       # [3,3] lattice vectors as rows
+    types: np.ndarray = None              # [N] int, default all 0
+    velocities: np.ndarray = None         # [N,3] A/ps; None -> Maxwell-Boltzmann at t_init
+    dt: float = 0.001                     # ps
+    ensemble: str = "NVE"                 # NVE | NVT | NPT
+    temperature: float = 300.0            # Langevin target, K
+    t_init: float = None                  # velocity initialisation, K (default temperature)
+    gamma: float = 10.0                   # Langevin friction, 1/ps
+    target_pressure: float = 1.0          # bar
+    tau_p: float = 1.0                    # ps
+    kappa: float = 4.57e-5                # 1/bar (barostat.py:27)
+    mu_max: float = 0.05                  # barostat.py:28
+    seed: int = 0
+
+
+class ReplicaEngine:
+    """Packed multi-replica MD state and step on one GPU."""
+
+    def __init__(self, specs, masses_amu, cutoff, skin, device="cuda", max_nbr=None,
+                 precise_list=True, seed=12345, velocity_seed=None):
+        self.device = torch.device(device)
+        if isinstance(specs, ReplicaSpec):
+            specs = [specs]
+        self.specs = list(specs)
+        self.R = len(self.specs)
+        self.masses_amu = [float(m) for m in masses_amu]
+        if len(self.masses_amu) > _lib.SIM_MAX_TYPES:
+            raise ValueError(f"at most {_lib.SIM_MAX_TYPES} atom types")
+        self.cutoff = float(cutoff)
+        self.skin = float(skin)
+        self.precise_list = bool(precise_list)
+        self.seed = int(seed)
+        _lib.load()
+        self._layout()
+        self._alloc(max_nbr, velocity_seed)
+        self.force_field = None
+        self._graph = None
+        self._graph_virial = None
+        self.steps_done = 0
+        self.profile = {"graph_replays": 0, "eager_steps": 0}
+
+    # ---- layout --------------------------------------------------------------
+    def _layout(self):
+        R = self.R
+        self.natoms = [int(len(s.positions)) for s in self.specs]
+        self.atom_base = np.concatenate([[0], np.cumsum(self.natoms)]).astype(np.int64)
+        self.N = int(self.atom_base[-1])
+        if self.N >= 2 ** 31 - 1:
+            raise ValueError("too many atoms for int32 indices")
+        tiles, rep_tiles = [], [0]
+        for r in range(R):
+            for k in range(0, self.natoms[r], _lib.SIM_TILE):
+                tiles.append((r, int(self.atom_base[r]) + k))
+            rep_tiles.append(len(tiles))
+        self.ntiles = len(tiles)
+        self.tiles_np = np.array(tiles, np.int32).reshape(-1, 2)
+        self.rep_tiles_np = np.array(rep_tiles, np.int32)
+        rl2, rl = list_radius_sq(self.cutoff, self.skin)
+        self.rl2, self.rl = rl2, rl
+        caps, self.cell_base = [], [0]
+        for s in self.specs:
+            w = perpendicular_widths(s.box)
+            if s.ensemble.upper() == "NPT":
+                w = w * 1.25
+            nc = np.maximum(1, np.floor(w / (rl * (1 + 1e-6)))).astype(np.int64)
+            nc = np.minimum(nc, 1024)
+            caps.append(int(nc.prod()))
+            self.cell_base.append(self.cell_base[-1] + caps[-1])
+        self.cell_cap = caps
+        self.ncells = int(self.cell_base[-1])
+        self.cell_rep_np = np.repeat(np.arange(R, dtype=np.int32), caps)
+
+    def _replica_struct(self, r):
+        s = self.specs[r]
+        P = _lib.Replica()
+        H = np.asarray(s.box, np.float64)
+        Hinv = torch.linalg.inv(torch.as_tensor(H, dtype=torch.float64)).numpy()
+        for k in range(9):
+            P.h[k] = float(H.flat[k])
+            P.hinv[k] = float(Hinv.flat[k])
+        off = H - np.diag(np.diag(H))
+        P.ortho = int(bool(np.all(np.abs(off) <= 1e-8)))
+        P.volume = float(abs(np.linalg.det(H)))
+        P.rl2, P.rl = self.rl2, self.rl
+        half = np.float32(self.skin) / np.float32(2.0)
+        P.trig2 = float(np.float64(half) ** 2)
+        P.skin = float(np.float32(self.skin))
+        P.mu_acc = 1.0
+        P.dt = float(s.dt)
+        P.dtf = float(np.float32(s.dt))
+        P.mu = 1.0
+        P.target_p, P.tau_p, P.kappa, P.mu_max = s.target_pressure, s.tau_p, s.kappa, s.mu_max
+        ens = s.ensemble.upper()
+        P.langevin = int(ens in ("NVT", "NPT"))
+        P.barostat = int(ens == "NPT")
+        damp = np.float32(1.0)
+        for t, m in enumerate(self.masses_amu):
+            mi = internal_mass_f32(m)
+            if P.langevin:
+                damp, rff = langevin_f32(mi, s.dt, s.gamma, s.temperature)
+                P.rff[t] = float(rff)
+        P.damp = float(damp)
+        P.rcf = float(np.float32(self.cutoff))
+        P.atom_base = int(self.atom_base[r])
+        P.natoms = self.natoms[r]
+        P.cell_base = int(self.cell_base[r])
+        P.cell_cap = int(self.cell_cap[r])
+        return P
+
+    def _alloc(self, max_nbr, velocity_seed):
+        dev, N, R = self.device, self.N, self.R
+        i32 = dict(dtype=torch.int32, device=dev)
+        pos = np.zeros((N, 4), np.float64)
+        vel = np.zeros((N, 3), np.float32)
+        rng = np.random.default_rng(self.seed if velocity_seed is None else velocity_seed)
+        for r, s in enumerate(self.specs):
+            b, e = self.atom_base[r], self.atom_base[r + 1]
+            pos[b:e, :3] = np.asarray(s.positions, np.float64)
+            t = np.zeros(self.natoms[r], np.int64) if s.types is None else np.asarray(s.types)
+            if t.size and (t.min() < 0 or t.max() >= len(self.masses_amu)):
+                raise ValueError("atom type out of range")
+            pos[b:e, 3] = t
+            if s.velocities is not None:
+                vel[b:e] = np.asarray(s.velocities, np.float32)
+            else:
+                T0 = s.temperature if s.t_init is None else s.t_init
+                m = np.array([self.masses_amu[k] for k in t], np.float64) * MB_MASS_CONV
+                v = rng.standard_normal((len(t), 3)) * np.sqrt(KB * T0 / m)[:, None]
+                v -= (v * m[:, None]).sum(0) / m.sum()
+                vel[b:e] = v.astype(np.float32)
+        self.pos4 = torch.from_numpy(pos).to(dev)
+        self.pos_last = self.pos4.clone()
+        self.vel = torch.from_numpy(vel).to(dev)
+        self.vel_half = torch.zeros_like(self.vel)
+        self.force = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        self.tiles = torch.from_numpy(self.tiles_np).to(dev)
+        self.rep_tiles = torch.from_numpy(self.rep_tiles_np).to(dev)
+        self.reps = _lib.struct_tensor([self._replica_struct(r) for r in range(R)], dev)
+        self.cell_rep = torch.from_numpy(self.cell_rep_np).to(dev)
+        self.cell_count = torch.zeros(self.ncells + 1, **i32)
+        self.cell_start = torch.zeros(self.ncells + 1, **i32)
+        self.cell_atoms = torch.zeros(N, **i32)
+        self.atom_cell = torch.zeros(N, **i32)
+        self.atom_slot = torch.zeros(N, **i32)
+        if max_nbr is None:
+            est = 0.0
+            for r, s in enumerate(self.specs):
+                rho = self.natoms[r] / abs(np.linalg.det(np.asarray(s.box, np.float64)))
+                est = max(est, 4.0 / 3.0 * math.pi * self.rl ** 3 * rho)
+            max_nbr = int(math.ceil((1.3 * est + 16) / 8.0) * 8)
+        self.max_nbr = int(max_nbr)
+        self.nbr = torch.zeros((self.max_nbr, N), **i32)
+        self.nnbr = torch.zeros(N, **i32)
+        self.overflow = torch.zeros(1, **i32)
+        self.flag = torch.ones(R, **i32)
+        self.nbuild = torch.zeros(R, **i32)
+        self.maxd2 = torch.zeros(R, dtype=torch.float64, device=dev)
+        self.partial = torch.zeros((self.ntiles, _lib.SIM_NPART), dtype=torch.float64, device=dev)
+        self.thermo = torch.zeros((R, _lib.SIM_NPART), dtype=torch.float64, device=dev)
+        self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.types_struct = _lib.Types()
+        for t, m in enumerate(self.masses_amu):
+            self.types_struct.mass[t] = float(internal_mass_f32(m))
+        self.types_struct.ntypes = len(self.masses_amu)
+
+    # ---- primitives (each a stream-ordered launch) -----------------------------
+    def build_list(self, force=False):
+        """Skin trigger (force=True: every replica) and the gated rebuild."""
+        s = stream_ptr()
+        call("sim_nl_trigger", self.R, ptr(self.reps), ptr(self.maxd2), ptr(self.flag),
+             ptr(self.nbuild), int(bool(force)), s)
+        call("sim_nl_build", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
+             ptr(self.pos4), ptr(self.pos_last), ptr(self.flag), ptr(self.cell_rep), self.ncells,
+             ptr(self.atom_cell), ptr(self.atom_slot), ptr(self.cell_count), ptr(self.cell_start),
+             ptr(self.cell_atoms), ptr(self.nbr), ptr(self.nnbr), self.max_nbr,
+             ptr(self.overflow), int(self.precise_list), s)
+
+    def compute_forces(self, virial=True):
+        if self.force_field is None:
+            raise RuntimeError("no force field attached")
+        self.force_field.compute(self, virial)
+
+    def first_half(self):
+        call("sim_integrate_first", ptr(self.reps), ptr(self.tiles), self.ntiles, self.types_struct,
+             ptr(self.pos4), ptr(self.vel), ptr(self.force), ptr(self.vel_half),
+             ptr(self.pos_last), ptr(self.maxd2), stream_ptr())
+
+    def second_half(self, noise=None):
+        s = stream_ptr()
+        call("sim_integrate_second", ptr(self.reps), ptr(self.tiles), self.ntiles,
+             self.types_struct, ptr(self.pos4), ptr(self.vel_half), ptr(self.force), ptr(self.vel),
+             ptr(noise), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
+        call("sim_step_advance", ptr(self.step_ctr), s)
+
+    def reduce(self, col0=0, col1=15):
+        call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), col0, col1,
+             ptr(self.thermo), stream_ptr())
+
+    def barostat(self):
+        call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
+             ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last), stream_ptr())
+
+    @property
+    def has_barostat(self):
+        return any(s.ensemble.upper() == "NPT" for s in self.specs)
+
+    # ---- step ----------------------------------------------------------------
+    def setup(self):
+        """Initial list and forces at the current positions (md_model.py:91-102)."""
+        self.build_list(force=True)
+        self.compute_forces(virial=True)
+        self.reduce(0, 8)
+        self._ke_now()
+
+    def _ke_now(self):
+        """Kinetic columns of thermo from the current velocities (setup only)."""
+        m = torch.tensor([float(x) for x in self.types_struct.mass[:len(self.masses_amu)]],
+                         dtype=torch.float64, device=self.device)
+        mi = m[self.pos4[:, 3].long()]
+        v = self.vel.double()
+        for r in range(self.R):
+            b, e = int(self.atom_base[r]), int(self.atom_base[r + 1])
+            vv, mm = v[b:e], mi[b:e, None]
+            self.thermo[r, _lib.T_KE] = 0.5 * (mm * vv * vv).sum()
+            K = (mm * vv).T @ vv
+            for k, (a, c) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+                self.thermo[r, _lib.T_K + k] = K[a, c]
+
+    def step(self, virial=None, noise=None):
+        """One MD step for every replica (md_model.py:103-153)."""
+        if virial is None:
+            virial = True
+        self.first_half()
+        self.build_list(force=False)
+        self.compute_forces(virial=virial or self.has_barostat)
+        self.second_half(noise)
+        self.reduce(0, 15)
+        if self.has_barostat:
+            self.barostat()
+        self.steps_done += 1
+
+    def capture(self, virial=True):
+        """Record one step into a HIP graph. Recording executes nothing, so the
+        state is unchanged until the first replay."""
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            self.step(virial=virial)
+        self.steps_done -= 1
+        self._graph, self._graph_virial = g, virial
+        torch.cuda.synchronize()
+        return g
+
+    def run(self, nsteps, graph=True, virial=True, check=True):
+        if graph:
+            if self._graph is None or self._graph_virial != virial:
+                self.capture(virial)
+            for _ in range(nsteps):
+                self._graph.replay()
+            self.steps_done += nsteps
+            self.profile["graph_replays"] += nsteps
+        else:
+            for _ in range(nsteps):
+                self.step(virial=virial)
+            self.profile["eager_steps"] += nsteps
+        if check:
+            self.check()
+
+    def check(self):
+        """Raise if a neighbor list overflowed (host read; call between chunks)."""
+        if int(self.overflow.item()) != 0:
+            raise RuntimeError(f"neighbor list overflow: an atom has more than {self.max_nbr} "
+                               "neighbors; construct the engine with a larger max_nbr")
+
+    # ---- host views ------------------------------------------------------------
+    def positions(self, dtype=torch.float64):
+        return self.pos4[:, :3].to(dtype)
+
+    def set_positions(self, xyz):
+        xyz = torch.as_tensor(xyz, device=self.device)
+        self.pos4[:, :3] = xyz.to(torch.float64)
+
+    def replica_slice(self, r):
+        return slice(int(self.atom_base[r]), int(self.atom_base[r + 1]))
+
+    def replica_state(self, r=0):
+        return _lib.struct_from_tensor(self.reps, _lib.Replica, r)
+
+    def box(self, r=0):
+        return np.array(self.replica_state(r).h[:], np.float64).reshape(3, 3)
+
+    def thermo_host(self):
+        return self.thermo.cpu().numpy()
+
+    def energies(self):
+        th = self.thermo_host()
+        return th[:, _lib.T_EPOT], th[:, _lib.T_KE]
+
+    def virial_tensor(self, r=0):
+        th = self.thermo[r]
+        w = th[_lib.T_W:_lib.T_W + 6]
+        return torch.stack([torch.stack([w[0], w[3], w[4]]), torch.stack([w[3], w[1], w[5]]),
+                            torch.stack([w[4], w[5], w[2]])])
+
+    def kinetic_tensor(self, r=0):
+        th = self.thermo[r]
+        k = th[_lib.T_K:_lib.T_K + 6]
+        return torch.stack([torch.stack([k[0], k[3], k[4]]), torch.stack([k[3], k[1], k[5]]),
+                            torch.stack([k[4], k[5], k[2]])])
+
+    def half_list(self):
+        """Current list as the reference's half list: edge_index int64 [2,E] (i<j,
+        lexicographic) and float64 distances, per replica indices are global."""
+        return half_list_from_full(self.nbr, self.nnbr, self.max_nbr, self.reps[:ctypes_size()],
+                                   self.positions(torch.float64))
+
+
+def ctypes_size():
+    import ctypes
+    return ctypes.sizeof(_lib.Replica)
+
+
+def half_list_from_full(nbr, nnbr, max_nbr, rep_bytes, xyz):
+    """Full column-major list -> (edge_index [2,E] int64 sorted, distances)."""
+    n = xyz.shape[0]
+    dev = xyz.device
+    hc = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+    hs = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+    s = stream_ptr()
+    call("sim_nl_half_count", ptr(nbr), ptr(nnbr), n, max_nbr, ptr(hc), ptr(hs), s)
+    E = int(hs[n].item())
+    ei = torch.empty((2, E), dtype=torch.int64, device=dev)
+    dist = torch.empty(E, dtype=xyz.dtype, device=dev)
+    dtype = 0 if xyz.dtype == torch.float32 else 1
+    call("sim_nl_half_fill", ptr(rep_bytes), ptr(xyz.contiguous()), dtype, n, ptr(nbr), ptr(nnbr),
+         max_nbr, ptr(hs), ptr(ei) if E else None, E, ptr(dist) if E else None, s)
+    return ei, dist

@@ -1,0 +1,83 @@
+"""Force fields as engine plug-ins: each writes forces and the energy/virial
+partial columns of every replica in one or two kernel launches."""
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_ptr
+from .eam_parser import EAMParser, EAMTables
+
+LJ_MODES = {"raw": 0, "shift": 1, "switch": 2}
+
+
+class LJForceField:
+    """Lennard-Jones (core/force/lennard_jones_force.py:15-95).
+
+    eps, sig: per-type-pair tables [nt][nt] (eV, Angstrom); rc: cutoff;
+    mode: raw | shift (force shift, :65-75) | switch (:48-63, start at
+    switch_ratio * rc).
+    """
+    name = "lj"
+
+    def __init__(self, eps, sig, rc, mode="raw", switch_ratio=0.9):
+        eps = np.atleast_2d(np.asarray(eps, np.float64))
+        sig = np.atleast_2d(np.asarray(sig, np.float64))
+        nt = eps.shape[0]
+        if nt > _lib.SIM_MAX_TYPES or eps.shape != (nt, nt) or sig.shape != (nt, nt):
+            raise ValueError("eps/sig must be square per-type tables of at most 4 types")
+        p = _lib.LJParams()
+        for a in range(nt):
+            for b in range(nt):
+                p.eps[a * _lib.SIM_MAX_TYPES + b] = float(np.float32(eps[a, b]))
+                p.sig[a * _lib.SIM_MAX_TYPES + b] = float(np.float32(sig[a, b]))
+        p.rc = float(np.float32(rc))
+        p.rs = float(np.float32(switch_ratio * rc))
+        p.mode = LJ_MODES[mode] if isinstance(mode, str) else int(mode)
+        self.params = p
+        self.rc = float(rc)
+
+    def compute(self, eng, virial=True):
+        call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(eng.force), ptr(eng.partial),
+             int(bool(virial)), stream_ptr())
+
+
+class EAMForceField:
+    """Two-pass EAM (core/force/eam_force_cu.py:461-579) from an EAM file."""
+    name = "eam"
+
+    def __init__(self, source, n_r=8192, n_rho=4096, device="cuda"):
+        parser = source if isinstance(source, EAMParser) else EAMParser(source)
+        self.parser = parser
+        self.tables = EAMTables(parser, n_r, n_rho)
+        self.cutoff = self.tables.cutoff
+        self.dev = self.tables.device_arrays(device)
+        t = self.tables
+        p = _lib.EAMParams()
+        p.pass1 = self.dev["pass1"].data_ptr()
+        p.pass2 = self.dev["pass2"].data_ptr()
+        p.emb = self.dev["emb"].data_ptr()
+        p.demb = self.dev["demb"].data_ptr()
+        p.n_r, p.n_rho, p.ntypes = t.n_r, t.n_rho, t.ne
+        p.inv_dr = float(np.float32(t.inv_dr))
+        p.r_clamp = float(np.float32(t.r_max * (1 - 1e-7)))
+        p.cutoff = float(np.float32(t.cutoff))
+        for a in range(t.ne):
+            p.rho_lo[a] = float(np.float32(t.rho_min[a]))
+            p.rho_hi[a] = float(np.float32(t.rho_min[a] + t.drho[a] * (t.n_rho - 1) * (1 - 1e-7)))
+            p.inv_drho[a] = float(np.float32(t.inv_drho[a]))
+        self.params = p
+        self.rho = None
+        self.fp = None
+
+    def compute(self, eng, virial=True):
+        if self.rho is None or self.rho.numel() != eng.N or self.rho.device != eng.device:
+            self.rho = torch.zeros(eng.N, dtype=torch.float32, device=eng.device)
+            self.fp = torch.zeros(eng.N, dtype=torch.float32, device=eng.device)
+        s = stream_ptr()
+        call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(self.rho), ptr(self.fp),
+             ptr(eng.partial), s)
+        call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(self.fp), ptr(eng.force),
+             ptr(eng.partial), int(bool(virial)), s)

@@ -1,0 +1,241 @@
+"""GPU parity of the MD-step kernels against the CPU oracle (float64).
+
+Bars: neighbor pair sets identical; float32 forces within 1e-5 relative
+max-atom error of the float64 oracle on the same configuration; energies and
+virials within 1e-6 relative; integrator half steps reproduce the reference
+formulas with injected noise.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import force_error, oracle_eam, oracle_half_list, oracle_lj, pair_set, sym3
+from oracle import integrate as oint
+from simulon_amd import lattice as lat
+from simulon_amd.engine import ReplicaEngine, ReplicaSpec
+from simulon_amd.forcefield import EAMForceField, LJForceField
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+VIR_TOL = 1e-5
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _lj_engine(pos, H, ensemble="NVE", dt=0.002, **kw):
+    spec = ReplicaSpec(positions=pos, box=H, dt=dt, ensemble=ensemble, temperature=94.4, **kw)
+    eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
+    eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
+    return eng
+
+
+def test_library_version():
+    from simulon_amd import _lib
+    assert _lib.version().startswith("simulon_hip")
+
+
+def test_pairs_exact_lj_cubic():
+    pos, H = lat.lj_argon_nvt(seed=1)
+    _print_case("pairs_lj_cubic", n=len(pos))
+    eng = _lj_engine(pos, H)
+    eng.setup()
+    ei, d = eng.half_list()
+    ref, dref = oracle_half_list(pos, H, 8.5, 1.0)
+    assert pair_set(ei.cpu().numpy()) == pair_set(ref)
+    assert np.array_equal(ei.cpu().numpy(), ref)
+    np.testing.assert_array_equal(d.cpu().numpy(), dref)
+    eng.check()
+
+
+def test_pairs_exact_triclinic():
+    pos, H = lat.lj_argon_npt_tri(ncell=6, seed=2)
+    _print_case("pairs_triclinic", n=len(pos))
+    eng = _lj_engine(pos, H)
+    eng.setup()
+    ei, _ = eng.half_list()
+    ref, _ = oracle_half_list(pos, H, 8.5, 1.0)
+    assert np.array_equal(ei.cpu().numpy(), ref)
+
+
+def test_lj_forces_vs_fp64():
+    pos, H = lat.lj_argon_nvt(seed=0)
+    _print_case("lj_forces", n=len(pos))
+    eng = _lj_engine(pos, H)
+    eng.setup()
+    torch.cuda.synchronize()
+    ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
+    err = force_error(eng.force.cpu().numpy(), ref["forces"])
+    th = eng.thermo_host()[0]
+    print(f"force err {err:.3e}  E {th[0]:.9f} vs {ref['energy']:.9f}", flush=True)
+    assert err <= FORCE_TOL
+    assert abs(th[0] - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+    # per-atom virial terms are float32 sums of pair terms of both signs:
+    # same 1e-5 relative bar as the forces
+    assert abs(th[1] - ref["virial"]) <= VIR_TOL * abs(ref["virial"])
+    W = sym3(th[2:8])
+    assert np.abs(W - ref["virial_tensor"]).max() <= VIR_TOL * np.abs(ref["virial_tensor"]).max()
+
+
+@pytest.mark.parametrize("mode", ["shift", "switch"])
+def test_lj_smoothing_modes(mode):
+    pos, H = lat.lj_argon_nvt(seed=3)
+    spec = ReplicaSpec(positions=pos, box=H, dt=0.002)
+    eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
+    eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, mode=mode)
+    eng.setup()
+    ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0,
+                    mode=mode)
+    assert force_error(eng.force.cpu().numpy(), ref["forces"]) <= FORCE_TOL
+    th = eng.thermo_host()[0]
+    assert abs(th[0] - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+
+
+def test_lj_triclinic_forces():
+    pos, H = lat.lj_argon_npt_tri(ncell=8, seed=5)
+    _print_case("lj_triclinic", n=len(pos))
+    eng = _lj_engine(pos, H)
+    eng.setup()
+    ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
+    err = force_error(eng.force.cpu().numpy(), ref["forces"])
+    print(f"tri force err {err:.3e}", flush=True)
+    assert err <= FORCE_TOL
+
+
+@pytest.fixture(scope="module")
+def w_setfl(tmp_path_factory):
+    p = tmp_path_factory.mktemp("eam") / "W_synthetic.eam.alloy"
+    return str(lat.write_w_synthetic_setfl(p))
+
+
+def test_eam_forces_vs_fp64(w_setfl):
+    pos, H = lat.w_bcc(8, seed=7, amp=0.08)
+    _print_case("eam_forces", n=len(pos))
+    spec = ReplicaSpec(positions=pos, box=H, dt=0.001)
+    eng = ReplicaEngine([spec], [lat.W_MASS], cutoff=5.5, skin=1.0)
+    eng.force_field = EAMForceField(w_setfl)
+    eng.setup()
+    ref = oracle_eam(pos, H, np.zeros(len(pos), int), w_setfl, 1.0)
+    err = force_error(eng.force.cpu().numpy(), ref["forces"])
+    th = eng.thermo_host()[0]
+    print(f"eam force err {err:.3e} E {th[0]:.8f} vs {ref['energy']:.8f}", flush=True)
+    assert err <= FORCE_TOL
+    assert abs(th[0] - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+    assert abs(th[1] - ref["virial"]) <= VIR_TOL * abs(ref["virial"])
+    W = sym3(th[2:8])
+    assert np.abs(W - ref["virial_tensor"]).max() <= VIR_TOL * np.abs(ref["virial_tensor"]).max()
+
+
+def test_integrator_injected_noise():
+    pos, H = lat.lj_argon_nvt(seed=4)
+    eng = _lj_engine(pos, H, ensemble="NVT", gamma=10.0)
+    eng.setup()
+    x0 = eng.positions().cpu().numpy().copy()
+    v0 = eng.vel.cpu().numpy().copy()
+    f0 = eng.force.cpu().numpy().copy()
+    noise = torch.randn((eng.N, 3), generator=torch.Generator().manual_seed(9)).to(eng.device)
+    m = np.full(eng.N, np.float32(eng.types_struct.mass[0]), np.float32)
+    eng.step(noise=noise)
+    torch.cuda.synchronize()
+    vh, x1 = oint.first_half(x0, v0, f0, m, 0.002)
+    x1_64 = x0 + vh.astype(np.float64) * 0.002
+    np.testing.assert_array_equal(eng.vel_half.cpu().numpy(), vh)
+    np.testing.assert_allclose(eng.positions().cpu().numpy(), x1_64, rtol=0, atol=1e-12)
+    f1 = eng.force.cpu().numpy()
+    v1, ke, T = oint.second_half(vh, f1, m, 0.002, gamma=10.0, T=94.4, noise=noise.cpu().numpy())
+    np.testing.assert_allclose(eng.vel.cpu().numpy(), v1, rtol=2e-6, atol=1e-6)
+    th = eng.thermo_host()[0]
+    assert abs(th[8] - ke) <= 1e-5 * ke
+
+
+def test_batched_equals_single():
+    sizes = [3, 4, 5]
+    specs, singles = [], []
+    for k, n in enumerate(sizes):
+        pos, H = lat.w_bcc(2 * n, seed=10 + k, amp=0.05)
+        specs.append(ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT",
+                                 temperature=300 + 100 * k, gamma=5.0))
+    rng_v = [None] * len(specs)
+    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.0, skin=1.0)
+    eng.force_field = LJForceField([[0.1]], [[2.5]], 5.0)
+    eng.setup()
+    f_all = eng.force.cpu().numpy()
+    for k, s in enumerate(specs):
+        s2 = ReplicaSpec(positions=s.positions, box=s.box, velocities=eng.vel[eng.replica_slice(k)].cpu().numpy(),
+                         dt=0.001)
+        e1 = ReplicaEngine([s2], [lat.W_MASS], cutoff=5.0, skin=1.0)
+        e1.force_field = LJForceField([[0.1]], [[2.5]], 5.0)
+        e1.setup()
+        np.testing.assert_array_equal(e1.force.cpu().numpy(), f_all[eng.replica_slice(k)])
+    del rng_v
+
+
+def test_nve_trajectory_vs_fp64_oracle():
+    """100 graph-replayed NVE steps (raw cutoff) follow the float64 oracle's
+    total-energy trajectory; the raw cutoff itself does not conserve energy."""
+    from oracle import lj as olj
+    from oracle import neighbor as onb
+    pos, H = lat.lj_argon_nvt(seed=6)
+    eng = _lj_engine(pos, H, ensemble="NVE", dt=0.002, t_init=94.4)
+    eng.setup()
+    x = eng.positions().cpu().numpy().copy()
+    v = eng.vel.cpu().numpy().astype(np.float64)
+    m = np.full(len(x), float(eng.types_struct.mass[0]))
+    rl2 = onb.list_radius_sq(8.5, 1.0)
+
+    def force(x):
+        ei, _ = onb.brute_force_half_list(x, H, rl2)
+        return olj.lj_forward(x, ei, np.zeros(len(x), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, H)
+
+    out = force(x)
+    f = out["forces"]
+    e_or = []
+    for _ in range(100):
+        vh = v + 0.5 * f / m[:, None] * 0.002
+        x = x + vh * 0.002
+        out = force(x)
+        f = out["forces"]
+        v = vh + 0.5 * f / m[:, None] * 0.002
+        e_or.append(out["energy"] + 0.5 * (m[:, None] * v * v).sum())
+    e_gpu = []
+    for _ in range(100):
+        eng.run(1, graph=True, check=False)
+        th = eng.thermo_host()[0]
+        e_gpu.append(th[0] + th[8])
+    eng.check()
+    d = np.max(np.abs(np.array(e_gpu) - np.array(e_or)))
+    print(f"NVE max |E_gpu - E_fp64| over 100 steps: {d:.3e} eV (drift {e_or[-1]-e_or[0]:.3e})",
+          flush=True)
+    assert d < 1e-3
+    assert int(eng.nbuild.cpu()[0]) >= 2
+
+
+def test_nve_conservation_force_shift():
+    pos, H = lat.lj_argon_nvt(seed=6)
+    spec = ReplicaSpec(positions=pos, box=H, dt=0.002, ensemble="NVE", t_init=94.4)
+    eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
+    eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, mode="shift")
+    eng.setup()
+    th = eng.thermo_host()[0]
+    E0 = th[0] + th[8]
+    eng.run(500, graph=True)
+    th = eng.thermo_host()[0]
+    E1 = th[0] + th[8]
+    print(f"force-shift NVE 1 ps: E0 {E0:.6f} E1 {E1:.6f} (KE {th[8]:.3f})", flush=True)
+    assert abs(E1 - E0) < 2e-3 * th[8]
+
+
+def test_graph_matches_eager():
+    pos, H = lat.lj_argon_nvt(seed=8)
+    a = _lj_engine(pos, H, ensemble="NVT", gamma=10.0)
+    b = _lj_engine(pos, H, ensemble="NVT", gamma=10.0)
+    b.vel.copy_(a.vel)
+    a.setup()
+    b.setup()
+    a.run(20, graph=False)
+    b.run(20, graph=True)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(a.pos4.cpu().numpy(), b.pos4.cpu().numpy())
+    np.testing.assert_array_equal(a.vel.cpu().numpy(), b.vel.cpu().numpy())
