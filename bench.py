@@ -1,0 +1,318 @@
+"""Benchmark: atom-steps/s of the MD step on one MI355X.
+
+Default workload (w_eam_replicas): 64 W EAM replicas of uneven size (2k..128k
+atoms, whole bcc cells, sizes drawn log-uniform with seed 3), Langevin at
+100..1500 K (gamma 10/ps), dt = 1 fs, synthetic setfl tables (5000 points:
+f = exp(-1.5(r-2.74)), phi = 0.6[exp(-4(r-2.74)) - 2 exp(-2(r-2.74))],
+F = -2.5 sqrt(rho), cubic taper 5.0-5.5 A), cutoff 5.5 A, skin 1.0 A.
+
+  batched  all replicas in one packed engine, one graph replay per step
+  plain    the same replicas advanced one after another, each by its own
+           single-system engine (graph-replayed too), as a per-replica scan does
+
+`value` is the batched whole-job atom-steps/s. Also reported: the plain-loop
+rate, the speedup, ns/day, the float32 max-atom force error against the
+float64 oracle on sampled replicas, the roofline of the dominant kernel
+(EAM force pass, timed with HIP events on its stream), and the oracle's CPU
+rate on a bounded sample (host cores).
+
+Other workloads (--workload): lj_argon_nvt (1000 Ar), w_eam_nve (54,000 W),
+lj_argon_npt_tri (1,048,576 Ar, triclinic, Berendsen + Langevin).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from simulon_amd import lattice as lat  # noqa: E402
+from simulon_amd.engine import ReplicaEngine, ReplicaSpec  # noqa: E402
+from simulon_amd.forcefield import EAMForceField, LJForceField  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "atom-steps/s on 1 GPU: plain vs 64-replica batched EAM; force err vs FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def mean_full_list(eng):
+    return float(eng.nnbr.double().mean().item())
+
+
+def kernel_time_ms(launch, reps=50):
+    """Average duration of one launch, HIP events on the launching stream."""
+    launch()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        launch()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def eam_kernel_roofline(eng):
+    """Roofline of the EAM passes with the algorithmic bytes per atom:
+    pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
+    read) + 12 (F write). The dominant (longer) pass is reported."""
+    from simulon_amd._lib import call, ptr, stream_ptr
+    ff = eng.force_field
+    nbar = mean_full_list(eng)
+
+    rho = eng.scratch("eam_rho", torch.float32)
+    fp = eng.scratch("eam_fp", torch.float32)
+
+    def p1():
+        call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(rho), ptr(fp),
+             ptr(eng.partial), stream_ptr())
+
+    def p2():
+        call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(fp), ptr(eng.force),
+             ptr(eng.partial), 1, stream_ptr())
+
+    t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
+    b1 = eng.N * (12 + 4 * nbar + 4)
+    b2 = eng.N * (12 + 4 * nbar + 4 + 12)
+    name, t, b = ("eam_force", t2, b2) if t2 >= t1 else ("eam_density", t1, b1)
+    ach = b / (t * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
+            "other_pass_ms": round(t1 if name == "eam_force" else t2, 5)}
+
+
+def lj_kernel_roofline(eng):
+    from simulon_amd._lib import call, ptr, stream_ptr
+    ff = eng.force_field
+    nbar = mean_full_list(eng)
+
+    def k():
+        call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force), ptr(eng.partial),
+             1, stream_ptr())
+
+    t = kernel_time_ms(k)
+    b = eng.N * (12 + 4 * nbar + 12)
+    ach = b / (t * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "lj_force", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2)}
+
+
+# ---------------------------------------------------------------------------
+def replica_specs(nrep=64):
+    sides = lat.replica_sizes(nrep)
+    temps = np.linspace(100.0, 1500.0, nrep)
+    specs = []
+    for k, (n, T) in enumerate(zip(sides, temps)):
+        pos, H = lat.w_bcc(int(n))
+        specs.append(ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT",
+                                 temperature=float(T), gamma=10.0))
+    return specs
+
+
+def force_error_sample(eng, setfl, reps):
+    """float32 GPU forces vs the float64 oracle on the replicas `reps`."""
+    from oracle import eam as oeam
+    from oracle import neighbor as onb
+    tab = oeam.EAMTables(oeam.EAMFile(setfl), dtype=np.float64)
+    eng.build_list(force=True)
+    eng.compute_forces(virial=True)
+    torch.cuda.synchronize()
+    worst = 0.0
+    f_all = eng.force.cpu().numpy()
+    x_all = eng.positions().cpu().numpy()
+    for r in reps:
+        sl = eng.replica_slice(r)
+        x = x_all[sl]
+        H = eng.box(r)
+        ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(5.5, 1.0))
+        ref = oeam.eam_forward(x, ei, np.zeros(len(x), int), tab, H)["forces"]
+        e = float(np.max(np.linalg.norm(f_all[sl] - ref, axis=1)) /
+                  np.max(np.linalg.norm(ref, axis=1)))
+        log(f"  force err replica {r} (N={len(x)}): {e:.3e}")
+        worst = max(worst, e)
+    return worst
+
+
+def cpu_baseline_eam(setfl, natoms_side=10, seconds=15.0):
+    """The oracle's float32 EAM step (edge list built once, as between list
+    rebuilds) on one replica-sized cell, timed on the host."""
+    from oracle import eam as oeam
+    from oracle import integrate as oint
+    from oracle import neighbor as onb
+    pos, H = lat.w_bcc(natoms_side, seed=1, amp=0.02)
+    x = pos.astype(np.float32)
+    n = len(x)
+    tab = oeam.EAMTables(oeam.EAMFile(setfl), dtype=np.float32)
+    ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(5.5, 1.0))
+    m = oint.internal_mass(np.full(n, lat.W_MASS))
+    rng = np.random.default_rng(0)
+    v = oint.maxwell_boltzmann(np.full(n, lat.W_MASS), 300.0, rng)
+    f = oeam.eam_forward(x, ei, np.zeros(n, int), tab, H)["forces"]
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        vh, x = oint.first_half(x, v, f, m, 0.001)
+        f = oeam.eam_forward(x, ei, np.zeros(n, int), tab, H)["forces"]
+        v, _, _ = oint.second_half(vh, f, m, 0.001, gamma=10.0, T=300.0,
+                                   noise=rng.standard_normal((n, 3)).astype(np.float32))
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(steps * n / dt, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle float32 EAM + BAOB step, {n}-atom W bcc cell, {steps} steps in "
+                      f"{dt:.1f} s, list built once (host {os.cpu_count()} cores visible, numpy "
+                      f"single-threaded)"}
+
+
+def bench_replicas(args):
+    tmp = tempfile.mkdtemp()
+    setfl = lat.write_w_synthetic_setfl(os.path.join(tmp, "W_synthetic.eam.alloy"))
+    specs = replica_specs(args.replicas)
+    natoms = np.array([len(s.positions) for s in specs])
+    log(f"replicas {len(specs)} atoms total {natoms.sum()} min {natoms.min()} max {natoms.max()}")
+    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024)
+    eng.force_field = EAMForceField(setfl, device=eng.device)
+    eng.setup()
+    eng.run(args.warmup, graph=True, virial=False)
+    t = timed(lambda k: eng.run(k, graph=True, virial=False, check=False), args.steps)
+    eng.check()
+    batched = natoms.sum() * args.steps / t
+    builds = eng.nbuild.cpu().numpy()
+    log(f"batched: {t * 1e3 / args.steps:.3f} ms/step, {batched:.4e} atom-steps/s, "
+        f"rebuilds per replica min {builds.min()} max {builds.max()}")
+    roof = eam_kernel_roofline(eng)
+    log(f"roofline {roof}")
+    # plain per-replica loop on the same GPU
+    plain_t = 0.0
+    plain_steps = args.plain_steps or args.steps
+    for k, s in enumerate(specs):
+        e1 = ReplicaEngine([s], [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024 + k)
+        e1.force_field = eng.force_field
+        e1.setup()
+        e1.run(2, graph=True, virial=False)
+        plain_t += timed(lambda k_: e1.run(k_, graph=True, virial=False, check=False), plain_steps)
+        e1.check()
+        del e1
+    plain = natoms.sum() * plain_steps / plain_t
+    log(f"plain loop: {plain_t * 1e3 / plain_steps:.3f} ms per all-replica step, {plain:.4e} "
+        f"atom-steps/s, speedup {batched / plain:.2f}x")
+    small = int(np.argmin(natoms))
+    mid = int(np.argsort(natoms)[len(natoms) // 2])
+    ferr = force_error_sample(eng, setfl, [small, mid]) if not args.no_check else None
+    out = {
+        "metric": METRIC, "value": round(float(batched), 1), "unit": "atom-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "w_eam_replicas", "replicas": len(specs),
+                   "atoms_total": int(natoms.sum()), "atoms_min": int(natoms.min()),
+                   "atoms_max": int(natoms.max()), "dt_fs": 1.0, "temperatures_K": [100, 1500],
+                   "gamma_per_ps": 10.0, "cutoff": 5.5, "skin": 1.0, "size_seed": 3,
+                   "eam_table": "synthetic setfl, 5000 points"},
+        "plain_loop_atom_steps_per_s": round(float(plain), 1),
+        "speedup_vs_plain": round(float(batched / plain), 3),
+        "ns_per_day": round(args.steps * 0.001e-3 * 86400 / t, 3),
+        "force_err_vs_fp64": ferr,
+        "rebuilds_per_replica": [int(builds.min()), int(builds.max())],
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_eam(setfl, seconds=args.cpu_seconds)
+    return out
+
+
+def bench_single(args):
+    w = args.workload
+    if w == "lj_argon_nvt":
+        pos, H = lat.lj_argon_nvt(seed=0)
+        spec = ReplicaSpec(positions=pos, box=H, dt=0.002, ensemble="NVT", temperature=94.4,
+                           gamma=10.0)
+        eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
+        eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
+        dt_ps = 0.002
+    elif w == "lj_argon_npt_tri":
+        pos, H = lat.lj_argon_npt_tri(ncell=args.fcc_cells, seed=0)
+        spec = ReplicaSpec(positions=pos, box=H, dt=0.002, ensemble="NPT", temperature=90.0,
+                           gamma=10.0, target_pressure=1.0, tau_p=1.0)
+        eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
+        eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
+        dt_ps = 0.002
+    elif w == "w_eam_nve":
+        tmp = tempfile.mkdtemp()
+        setfl = lat.write_w_synthetic_setfl(os.path.join(tmp, "W_synthetic.eam.alloy"))
+        pos, H = lat.w_bcc(30)
+        spec = ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVE", temperature=300.0)
+        eng = ReplicaEngine([spec], [lat.W_MASS], cutoff=5.5, skin=1.0)
+        eng.force_field = EAMForceField(setfl, device=eng.device)
+        dt_ps = 0.001
+    else:
+        raise SystemExit(f"unknown workload {w}")
+    eng.setup()
+    virial = eng.has_barostat
+    eng.run(args.warmup, graph=True, virial=virial)
+    t = timed(lambda k: eng.run(k, graph=True, virial=virial, check=False), args.steps)
+    eng.check()
+    rate = eng.N * args.steps / t
+    roof = lj_kernel_roofline(eng) if isinstance(eng.force_field, LJForceField) else \
+        eam_kernel_roofline(eng)
+    return {"metric": f"atom-steps/s ({w})", "value": round(float(rate), 1), "unit": "atom-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(t * 1e3 / args.steps, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": w, "atoms": eng.N},
+            "ns_per_day": round(args.steps * dt_ps * 1e-3 * 86400 / t, 3),
+            "rebuilds": int(eng.nbuild.cpu()[0]), "roofline": roof}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="w_eam_replicas")
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=0)
+    ap.add_argument("--fcc-cells", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(replicas and domains shard across GPUs by design only; see DESIGN.md)",
+              file=sys.stderr)
+        raise SystemExit(2)
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    if args.workload == "w_eam_replicas":
+        out = bench_replicas(args)
+    else:
+        out = bench_single(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

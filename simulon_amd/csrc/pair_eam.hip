@@ -15,7 +15,7 @@
 // linear interpolation needs for two functions: pass1[k] = (f(k), f(k+1),
 // phi(k), phi(k+1)), pass2[k] = (f'(k), f'(k+1), phi'(k), phi'(k+1)), with
 // the k+1 entry clamped at the last grid point as the reference clamps `next`.
-#include "sim_common.h"
+#include "pair_common.h"
 
 namespace {
 
@@ -33,41 +33,86 @@ __device__ __forceinline__ Lerp lerp_r(float r, const sim_eam_params_t& E) {
 
 __device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t * (b - a); }
 
+// ---- pass 1 ---------------------------------------------------------------
+template <int NT, bool ORTHO>
+__device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const double4 xi, int cnt,
+                                             const BoxRegs& B, const double4* __restrict__ pos,
+                                             const int* __restrict__ nbr, int ns,
+                                             const sim_eam_params_t& E, float thr) {
+    constexpr int U = SIM_NB_BATCH;
+    const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
+    const int ti = (int)xi.w;
+    const int nt = NT == 1 ? 1 : E.ntypes;
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+        int j[U];
+        double4 xj[U];
+        float r2[U];
+        Lerp L[U];
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * ns + i];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float rx, ry, rz;
+            displacement<ORTHO>(xi, xj[u], B, rx, ry, rz);
+            r2[u] = sq3(rx, ry, rz);
+            L[u] = lerp_r(__builtin_sqrtf(r2[u]), E);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int tj = NT == 1 ? 0 : (int)xj[u].w;
+            v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (r2[u] < thr) {
+                rho += lerp2(v[u].x, v[u].y, L[u].frac);
+                ep += lerp2(v[u].z, v[u].w, L[u].frac);
+            }
+        }
+    }
+    for (; k < cnt; ++k) {
+        const int j = nbr[(size_t)k * ns + i];
+        const double4 xj = pos[j];
+        float rx, ry, rz;
+        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
+        const float r2 = sq3(rx, ry, rz);
+        if (r2 < thr) {
+            const int tj = NT == 1 ? 0 : (int)xj.w;
+            const Lerp L = lerp_r(__builtin_sqrtf(r2), E);
+            const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
+            rho += lerp2(v.x, v.y, L.frac);
+            ep += lerp2(v.z, v.w, L.frac);
+        }
+    }
+}
+
 template <int NT>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                   const double4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
                   float* __restrict__ rho_out, float* __restrict__ fp_out,
-                  double* __restrict__ partial, float rc2_thr) {
+                  double* __restrict__ partial, float thr) {
     __shared__ double lds[4 * 1];
     const SimTile t = tiles[blockIdx.x];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
-    const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
     double etot = 0.0;
     if (active) {
+        const BoxRegs B = load_box(P);
         const double4 xi = pos[i];
         const int ti = (int)xi.w;
         const int cnt = nnbr[i];
         float rho = 0.f, ep = 0.f;
-        for (int k = 0; k < cnt; ++k) {
-            const int j = nbr[(size_t)k * n_stride + i];
-            const double4 xj = pos[j];
-            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            min_image_fast(dx, dy, dz, P);
-            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
-            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-            if (r2 < rc2_thr) {
-                const int tj = NT == 1 ? 0 : (int)xj.w;
-                const int nt = NT == 1 ? 1 : E.ntypes;
-                const Lerp L = lerp_r(sqrtf(r2), E);
-                const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
-                rho += lerp2(v.x, v.y, L.frac);
-                ep += lerp2(v.z, v.w, L.frac);
-            }
-        }
+        if (P.ortho)
+            density_atom<NT, true>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, thr);
+        else
+            density_atom<NT, false>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, thr);
         // embedding, eam_force_cu.py:448-459
         const float lo = E.rho_lo[ti];
         const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
@@ -85,88 +130,136 @@ __global__ void __launch_bounds__(SIM_TILE)
     if (threadIdx.x == 0) partial[(size_t)blockIdx.x * SIM_NPART + SIM_T_EPOT] = v[0];
 }
 
+// ---- pass 2 ---------------------------------------------------------------
+struct ForceAcc {
+    float fx = 0.f, fy = 0.f, fz = 0.f, vir = 0.f;
+    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+};
+
+template <bool VIRIAL>
+__device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, float rz, float r,
+                                               float s) {
+    const float g = -s * __builtin_amdgcn_rcpf(r + 1e-8f);
+    a.fx = __builtin_fmaf(g, rx, a.fx);
+    a.fy = __builtin_fmaf(g, ry, a.fy);
+    a.fz = __builtin_fmaf(g, rz, a.fz);
+    if (VIRIAL) {
+        a.vir = __builtin_fmaf(s, r, a.vir);
+        const float gx = g * rx, gy = g * ry;
+        a.wxx = __builtin_fmaf(gx, rx, a.wxx);
+        a.wyy = __builtin_fmaf(gy, ry, a.wyy);
+        a.wzz = __builtin_fmaf(g * rz, rz, a.wzz);
+        a.wxy = __builtin_fmaf(gx, ry, a.wxy);
+        a.wxz = __builtin_fmaf(gx, rz, a.wxz);
+        a.wyz = __builtin_fmaf(gy, rz, a.wyz);
+    }
+}
+
+template <int NT, bool VIRIAL, bool ORTHO>
+__device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi, int cnt, float fpi,
+                                           const BoxRegs& B, const double4* __restrict__ pos,
+                                           const int* __restrict__ nbr, int ns,
+                                           const float* __restrict__ fp, const sim_eam_params_t& E,
+                                           float thr) {
+    constexpr int U = SIM_NB_BATCH;
+    const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
+    const int ti = (int)xi.w;
+    const int nt = NT == 1 ? 1 : E.ntypes;
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+        int j[U];
+        double4 xj[U];
+        float rx[U], ry[U], rz[U], r2[U], r[U], fpj[U];
+        Lerp L[U];
+        float4 va[U], vb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * ns + i];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fpj[u] = fp[j[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            displacement<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
+            r2[u] = sq3(rx[u], ry[u], rz[u]);
+            r[u] = __builtin_sqrtf(r2[u]);
+            L[u] = lerp_r(r[u], E);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (NT == 1) {
+                va[u] = T2[L[u].idx];
+            } else {
+                const int tj = (int)xj[u].w;
+                va[u] = T2[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
+                vb[u] = T2[(size_t)(tj * nt + ti) * E.n_r + L[u].idx];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (r2[u] < thr) {
+                const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
+                const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
+                const float s = fpi * dda + fpj[u] * ddb + lerp2(va[u].z, va[u].w, L[u].frac);
+                eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
+            }
+        }
+    }
+    for (; k < cnt; ++k) {
+        const int j = nbr[(size_t)k * ns + i];
+        const double4 xj = pos[j];
+        float rx, ry, rz;
+        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
+        const float r2 = sq3(rx, ry, rz);
+        if (r2 < thr) {
+            const float r = __builtin_sqrtf(r2);
+            const Lerp L = lerp_r(r, E);
+            const float fpj = fp[j];
+            const int tj = NT == 1 ? 0 : (int)xj.w;
+            const float4 va = T2[(size_t)(ti * nt + tj) * E.n_r + L.idx];
+            const float4 vb = NT == 1 ? va : T2[(size_t)(tj * nt + ti) * E.n_r + L.idx];
+            const float dda = lerp2(va.x, va.y, L.frac);
+            const float ddb = lerp2(vb.x, vb.y, L.frac);
+            const float s = fpi * dda + fpj * ddb + lerp2(va.z, va.w, L.frac);
+            eam_pair_force<VIRIAL>(a, rx, ry, rz, r, s);
+        }
+    }
+}
+
 template <int NT, bool VIRIAL>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                 const double4* __restrict__ pos, const int* __restrict__ nbr,
                 const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
                 const float* __restrict__ fp, float* __restrict__ force,
-                double* __restrict__ partial, float rc2_thr) {
+                double* __restrict__ partial, float thr) {
     __shared__ double lds[4 * 7];
     const SimTile t = tiles[blockIdx.x];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
-    const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
-    float fx = 0.f, fy = 0.f, fz = 0.f, vir = 0.f;
-    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    ForceAcc a;
     if (active) {
+        const BoxRegs B = load_box(P);
         const double4 xi = pos[i];
-        const int ti = (int)xi.w;
         const int cnt = nnbr[i];
         const float fpi = fp[i];
-        for (int k = 0; k < cnt; ++k) {
-            const int j = nbr[(size_t)k * n_stride + i];
-            const double4 xj = pos[j];
-            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            min_image_fast(dx, dy, dz, P);
-            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
-            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-            if (r2 < rc2_thr) {
-                const float r = sqrtf(r2);
-                const Lerp L = lerp_r(r, E);
-                const float fpj = fp[j];
-                float s;
-                if (NT == 1) {
-                    const float4 v = T2[L.idx];
-                    const float dd = lerp2(v.x, v.y, L.frac);
-                    s = fpi * dd + fpj * dd + lerp2(v.z, v.w, L.frac);
-                } else {
-                    const int tj = (int)xj.w;
-                    const int nt = E.ntypes;
-                    const float4 a = T2[(size_t)(ti * nt + tj) * E.n_r + L.idx];
-                    const float4 b = T2[(size_t)(tj * nt + ti) * E.n_r + L.idx];
-                    s = fpi * lerp2(a.x, a.y, L.frac) + fpj * lerp2(b.x, b.y, L.frac) +
-                        lerp2(a.z, a.w, L.frac);
-                }
-                const float g = -s / (r + 1e-8f);
-                fx = __builtin_fmaf(g, rx, fx);
-                fy = __builtin_fmaf(g, ry, fy);
-                fz = __builtin_fmaf(g, rz, fz);
-                if (VIRIAL) {
-                    vir = __builtin_fmaf(s, r, vir);
-                    const float gx = g * rx, gy = g * ry;
-                    wxx = __builtin_fmaf(gx, rx, wxx);
-                    wyy = __builtin_fmaf(gy, ry, wyy);
-                    wzz = __builtin_fmaf(g * rz, rz, wzz);
-                    wxy = __builtin_fmaf(gx, ry, wxy);
-                    wxz = __builtin_fmaf(gx, rz, wxz);
-                    wyz = __builtin_fmaf(gy, rz, wyz);
-                }
-            }
-        }
-        force[(size_t)i * 3 + 0] = fx;
-        force[(size_t)i * 3 + 1] = fy;
-        force[(size_t)i * 3 + 2] = fz;
+        if (P.ortho)
+            force_atom<NT, VIRIAL, true>(a, i, xi, cnt, fpi, B, pos, nbr, n_stride, fp, E, thr);
+        else
+            force_atom<NT, VIRIAL, false>(a, i, xi, cnt, fpi, B, pos, nbr, n_stride, fp, E, thr);
+        force[(size_t)i * 3 + 0] = a.fx;
+        force[(size_t)i * 3 + 1] = a.fy;
+        force[(size_t)i * 3 + 2] = a.fz;
     }
-    double v[7] = {0.5 * vir, 0.5 * wxx, 0.5 * wyy, 0.5 * wzz, 0.5 * wxy, 0.5 * wxz, 0.5 * wyz};
+    double v[7] = {0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
+                   0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
     block_sum<7>(v, lds);
     if (threadIdx.x == 0) {
         double* p = partial + (size_t)blockIdx.x * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
     }
-}
-
-float sqrt_threshold(float rc) {
-    float x = rc * rc;
-    while (sqrtf(x) < rc) x = nextafterf(x, 1e30f);
-    while (true) {
-        float y = nextafterf(x, 0.f);
-        if (sqrtf(y) >= rc) x = y;
-        else break;
-    }
-    return x;
 }
 
 }  // namespace

@@ -6,113 +6,139 @@
 //     (coalesced index loads), so the per-atom force is a register sum — no
 //     atomics, deterministic order (the reference's kernel issues 6 float
 //     atomics per edge, "cuda source/lj_energy_force_kernel.cu:109-116");
+//   * neighbors are taken SIM_NB_BATCH at a time with every index and
+//     position load of the batch in flight together;
 //   * the displacement is formed in float64 from float64 positions and
 //     rounded once to float32; everything after that is float32;
 //   * energy and virial are halved per atom (each pair is seen from both
 //     sides), reduced over the workgroup in float64 with wave shuffles, one
 //     partial per tile; sim_reduce adds the tiles with compensation.
-#include "sim_common.h"
+#include "pair_common.h"
 
 namespace {
+
+struct LJAcc {
+    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, vir = 0.f;
+    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+};
+
+template <bool VIRIAL, int MODE>
+__device__ __forceinline__ void lj_pair(LJAcc& a, float rx, float ry, float rz, float r2, float thr,
+                                        float eps, float sig, const sim_lj_params_t& lj) {
+    if (!(r2 < thr)) return;
+    const float rc = lj.rc;
+    const float inv_r2 = 1.0f / r2;
+    const float s2 = sig * sig * inv_r2;
+    const float s6 = s2 * s2 * s2;
+    const float s12 = s6 * s6;
+    float U = 4.0f * eps * (s12 - s6);
+    float fr = 24.0f * eps * inv_r2 * (2.0f * s12 - s6);  // Fmag / r
+    if (MODE == 1) {  // force shift, lennard_jones_force.py:65-75
+        const float r = sqrtf(r2);
+        const float src = sig / rc;
+        const float q6 = src * src * src * src * src * src;
+        const float q12 = q6 * q6;
+        const float Urc = 4.0f * eps * (q12 - q6);
+        const float Frc = 24.0f * eps * (1.0f / rc) * (2.0f * q12 - q6);
+        U = U - Urc + (r - rc) * Frc;
+        fr = fr - Frc / r;
+    } else if (MODE == 2) {  // switch, lennard_jones_force.py:48-63
+        const float r = sqrtf(r2);
+        const float rs = lj.rs;
+        if (r > rs) {
+            float x = (r - rs) / (rc - rs);
+            x = fminf(fmaxf(x, 0.f), 1.f);
+            const float x2 = x * x, x3 = x2 * x;
+            const float S = 1.0f - 10.0f * x3 + 15.0f * x3 * x - 6.0f * x3 * x2;
+            const float dS = (-30.0f * x2 + 60.0f * x3 - 30.0f * x2 * x2) / (rc - rs);
+            fr = fr * S - U * dS / r;
+            U = U * S;
+        }
+    }
+    a.fx = __builtin_fmaf(fr, rx, a.fx);
+    a.fy = __builtin_fmaf(fr, ry, a.fy);
+    a.fz = __builtin_fmaf(fr, rz, a.fz);
+    a.e += U;
+    if (VIRIAL) {
+        a.vir = __builtin_fmaf(fr, r2, a.vir);
+        const float gx = fr * rx, gy = fr * ry;
+        a.wxx = __builtin_fmaf(gx, rx, a.wxx);
+        a.wyy = __builtin_fmaf(gy, ry, a.wyy);
+        a.wzz = __builtin_fmaf(fr * rz, rz, a.wzz);
+        a.wxy = __builtin_fmaf(gx, ry, a.wxy);
+        a.wxz = __builtin_fmaf(gx, rz, a.wxz);
+        a.wyz = __builtin_fmaf(gy, rz, a.wyz);
+    }
+}
+
+template <bool VIRIAL, int MODE, bool ORTHO>
+__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const double4 xi, int cnt,
+                                        const BoxRegs& B, const double4* __restrict__ pos,
+                                        const int* __restrict__ nbr, int n_stride,
+                                        const sim_lj_params_t& lj, float thr) {
+    constexpr int U = SIM_NB_BATCH;
+    const int ti = (int)xi.w;
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+        int j[U];
+        double4 xj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * n_stride + i];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float rx, ry, rz;
+            displacement<ORTHO>(xi, xj[u], B, rx, ry, rz);
+            const int tj = (int)xj[u].w;
+            lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr,
+                                  lj.eps[ti * SIM_MAX_TYPES + tj], lj.sig[ti * SIM_MAX_TYPES + tj],
+                                  lj);
+        }
+    }
+    for (; k < cnt; ++k) {
+        const int j = nbr[(size_t)k * n_stride + i];
+        const double4 xj = pos[j];
+        float rx, ry, rz;
+        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
+        const int tj = (int)xj.w;
+        lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, lj.eps[ti * SIM_MAX_TYPES + tj],
+                              lj.sig[ti * SIM_MAX_TYPES + tj], lj);
+    }
+}
 
 template <bool VIRIAL, int MODE>
 __global__ void __launch_bounds__(SIM_TILE)
     k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
          const double4* __restrict__ pos, const int* __restrict__ nbr,
          const int* __restrict__ nnbr, int n_stride, sim_lj_params_t lj,
-         float* __restrict__ force, double* __restrict__ partial, float rc2_thr) {
+         float* __restrict__ force, double* __restrict__ partial, float thr) {
     __shared__ double lds[4 * 8];
     const SimTile t = tiles[blockIdx.x];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
-    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, vir = 0.f;
-    float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    LJAcc a;
     if (active) {
+        const BoxRegs B = load_box(P);
         const double4 xi = pos[i];
-        const int ti = (int)xi.w;
         const int cnt = nnbr[i];
-        const float rc = lj.rc;
-        for (int k = 0; k < cnt; ++k) {
-            const int j = nbr[(size_t)k * n_stride + i];
-            const double4 xj = pos[j];
-            double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            min_image_fast(dx, dy, dz, P);
-            const float rx = (float)dx, ry = (float)dy, rz = (float)dz;
-            const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-            if (r2 < rc2_thr) {
-                const int tj = (int)xj.w;
-                const float eps = lj.eps[ti * SIM_MAX_TYPES + tj];
-                const float sig = lj.sig[ti * SIM_MAX_TYPES + tj];
-                const float inv_r2 = 1.0f / r2;
-                const float s2 = sig * sig * inv_r2;
-                const float s6 = s2 * s2 * s2;
-                const float s12 = s6 * s6;
-                float U = 4.0f * eps * (s12 - s6);
-                float fr = 24.0f * eps * inv_r2 * (2.0f * s12 - s6);  // Fmag / r
-                if (MODE == 1) {  // force shift, lennard_jones_force.py:65-75
-                    const float r = sqrtf(r2);
-                    const float src = sig / rc;
-                    const float q6 = src * src * src * src * src * src;
-                    const float q12 = q6 * q6;
-                    const float Urc = 4.0f * eps * (q12 - q6);
-                    const float Frc = 24.0f * eps * (1.0f / rc) * (2.0f * q12 - q6);
-                    U = U - Urc + (r - rc) * Frc;
-                    fr = fr - Frc / r;
-                } else if (MODE == 2) {  // switch, lennard_jones_force.py:48-63
-                    const float r = sqrtf(r2);
-                    const float rs = lj.rs;
-                    if (r > rs) {
-                        float x = (r - rs) / (rc - rs);
-                        x = fminf(fmaxf(x, 0.f), 1.f);
-                        const float x2 = x * x, x3 = x2 * x;
-                        const float S = 1.0f - 10.0f * x3 + 15.0f * x3 * x - 6.0f * x3 * x2;
-                        const float dS = (-30.0f * x2 + 60.0f * x3 - 30.0f * x2 * x2) / (rc - rs);
-                        fr = fr * S - U * dS / r;
-                        U = U * S;
-                    }
-                }
-                fx = __builtin_fmaf(fr, rx, fx);
-                fy = __builtin_fmaf(fr, ry, fy);
-                fz = __builtin_fmaf(fr, rz, fz);
-                e += U;
-                if (VIRIAL) {
-                    vir = __builtin_fmaf(fr, r2, vir);
-                    const float gx = fr * rx, gy = fr * ry;
-                    wxx = __builtin_fmaf(gx, rx, wxx);
-                    wyy = __builtin_fmaf(gy, ry, wyy);
-                    wzz = __builtin_fmaf(fr * rz, rz, wzz);
-                    wxy = __builtin_fmaf(gx, ry, wxy);
-                    wxz = __builtin_fmaf(gx, rz, wxz);
-                    wyz = __builtin_fmaf(gy, rz, wyz);
-                }
-            }
-        }
-        force[(size_t)i * 3 + 0] = fx;
-        force[(size_t)i * 3 + 1] = fy;
-        force[(size_t)i * 3 + 2] = fz;
+        if (P.ortho)
+            lj_atom<VIRIAL, MODE, true>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+        else
+            lj_atom<VIRIAL, MODE, false>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+        force[(size_t)i * 3 + 0] = a.fx;
+        force[(size_t)i * 3 + 1] = a.fy;
+        force[(size_t)i * 3 + 2] = a.fz;
     }
-    double v[8] = {0.5 * e, 0.5 * vir, 0.5 * wxx, 0.5 * wyy, 0.5 * wzz,
-                   0.5 * wxy, 0.5 * wxz, 0.5 * wyz};
+    double v[8] = {0.5 * a.e, 0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
+                   0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
     block_sum<8>(v, lds);
     if (threadIdx.x == 0) {
         double* p = partial + (size_t)blockIdx.x * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 8; ++k) p[k] = v[k];
     }
-}
-
-// Smallest float x with sqrtf(x) >= rc: r2 < thr  <=>  sqrtf(r2) < rc, so the
-// squared test reproduces the reference's `r < rc` mask exactly.
-float sqrt_threshold(float rc) {
-    float x = rc * rc;
-    while (sqrtf(x) < rc) x = nextafterf(x, 1e30f);
-    while (true) {
-        float y = nextafterf(x, 0.f);
-        if (sqrtf(y) >= rc) x = y;
-        else break;
-    }
-    return x;
 }
 
 template <bool V>

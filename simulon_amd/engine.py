@@ -62,6 +62,20 @@ def list_radius_sq(cutoff, skin):
     return float(np.float32(rl * rl)), float(rl)
 
 
+def morton_order(pos, H, bits=10):
+    """Permutation sorting atoms along a Z-order curve of their wrapped
+    fractional coordinates (bits per dimension), so that atoms close in
+    space are close in memory."""
+    s = np.asarray(pos, np.float64) @ np.linalg.inv(np.asarray(H, np.float64))
+    s = s - np.floor(s)
+    q = np.minimum((s * (1 << bits)).astype(np.uint64), (1 << bits) - 1)
+    key = np.zeros(len(q), np.uint64)
+    for b in range(bits):
+        for d in range(3):
+            key |= ((q[:, d] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + d)
+    return np.argsort(key, kind="stable")
+
+
 def perpendicular_widths(H):
     H = np.asarray(H, np.float64)
     V = abs(np.linalg.det(H))
@@ -93,7 +107,7 @@ class ReplicaEngine:
     """Packed multi-replica MD state and step on one GPU."""
 
     def __init__(self, specs, masses_amu, cutoff, skin, device="cuda", max_nbr=None,
-                 precise_list=True, seed=12345, velocity_seed=None):
+                 precise_list=True, seed=12345, velocity_seed=None, sort=True):
         self.device = torch.device(device)
         if isinstance(specs, ReplicaSpec):
             specs = [specs]
@@ -106,10 +120,12 @@ class ReplicaEngine:
         self.skin = float(skin)
         self.precise_list = bool(precise_list)
         self.seed = int(seed)
+        self.sort = bool(sort)
         _lib.load()
         self._layout()
         self._alloc(max_nbr, velocity_seed)
         self.force_field = None
+        self._scratch = {}
         self._graph = None
         self._graph_virial = None
         self.steps_done = 0
@@ -188,22 +204,32 @@ class ReplicaEngine:
         i32 = dict(dtype=torch.int32, device=dev)
         pos = np.zeros((N, 4), np.float64)
         vel = np.zeros((N, 3), np.float32)
+        order = np.arange(N, dtype=np.int64)
         rng = np.random.default_rng(self.seed if velocity_seed is None else velocity_seed)
         for r, s in enumerate(self.specs):
             b, e = self.atom_base[r], self.atom_base[r + 1]
-            pos[b:e, :3] = np.asarray(s.positions, np.float64)
+            x = np.asarray(s.positions, np.float64)
             t = np.zeros(self.natoms[r], np.int64) if s.types is None else np.asarray(s.types)
             if t.size and (t.min() < 0 or t.max() >= len(self.masses_amu)):
                 raise ValueError("atom type out of range")
-            pos[b:e, 3] = t
             if s.velocities is not None:
-                vel[b:e] = np.asarray(s.velocities, np.float32)
+                v = np.asarray(s.velocities, np.float32)
             else:
                 T0 = s.temperature if s.t_init is None else s.t_init
                 m = np.array([self.masses_amu[k] for k in t], np.float64) * MB_MASS_CONV
                 v = rng.standard_normal((len(t), 3)) * np.sqrt(KB * T0 / m)[:, None]
                 v -= (v * m[:, None]).sum(0) / m.sum()
-                vel[b:e] = v.astype(np.float32)
+                v = v.astype(np.float32)
+            perm = morton_order(x, s.box) if self.sort else np.arange(len(x))
+            pos[b:e, :3] = x[perm]
+            pos[b:e, 3] = t[perm]
+            vel[b:e] = v[perm]
+            order[b:e] = b + perm
+        # order[k]: caller's index of the atom stored in slot k
+        self.order = torch.from_numpy(order).to(dev)
+        inv = np.empty_like(order)
+        inv[order] = np.arange(N)
+        self.slot_of = torch.from_numpy(inv).to(dev)
         self.pos4 = torch.from_numpy(pos).to(dev)
         self.pos_last = self.pos4.clone()
         self.vel = torch.from_numpy(vel).to(dev)
@@ -238,6 +264,18 @@ class ReplicaEngine:
         for t, m in enumerate(self.masses_amu):
             self.types_struct.mass[t] = float(internal_mass_f32(m))
         self.types_struct.ntypes = len(self.masses_amu)
+
+    def scratch(self, name, dtype, shape=None):
+        """Per-engine work array owned by the engine (allocated on first use,
+        outside any graph capture, and kept for the engine's lifetime)."""
+        shape = (self.N,) if shape is None else tuple(shape)
+        t = self._scratch.get(name)
+        if t is None or t.dtype != dtype or tuple(t.shape) != shape:
+            if torch.cuda.is_current_stream_capturing():
+                raise RuntimeError(f"scratch array {name!r} first requested during graph capture")
+            t = torch.zeros(shape, dtype=dtype, device=self.device)
+            self._scratch[name] = t
+        return t
 
     # ---- primitives (each a stream-ordered launch) -----------------------------
     def build_list(self, force=False):
@@ -294,13 +332,13 @@ class ReplicaEngine:
                          dtype=torch.float64, device=self.device)
         mi = m[self.pos4[:, 3].long()]
         v = self.vel.double()
-        for r in range(self.R):
-            b, e = int(self.atom_base[r]), int(self.atom_base[r + 1])
-            vv, mm = v[b:e], mi[b:e, None]
-            self.thermo[r, _lib.T_KE] = 0.5 * (mm * vv * vv).sum()
-            K = (mm * vv).T @ vv
-            for k, (a, c) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
-                self.thermo[r, _lib.T_K + k] = K[a, c]
+        prods = torch.stack([v[:, 0] * v[:, 0], v[:, 1] * v[:, 1], v[:, 2] * v[:, 2],
+                             v[:, 0] * v[:, 1], v[:, 0] * v[:, 2], v[:, 1] * v[:, 2]], 1) * mi[:, None]
+        rep = torch.repeat_interleave(torch.arange(self.R, device=self.device),
+                                      torch.as_tensor(self.natoms, device=self.device))
+        K = torch.zeros((self.R, 6), dtype=torch.float64, device=self.device).index_add_(0, rep, prods)
+        self.thermo[:, _lib.T_K:_lib.T_K + 6] = K
+        self.thermo[:, _lib.T_KE] = 0.5 * (K[:, 0] + K[:, 1] + K[:, 2])
 
     def step(self, virial=None, noise=None):
         """One MD step for every replica (md_model.py:103-153)."""
@@ -349,6 +387,14 @@ class ReplicaEngine:
                                "neighbors; construct the engine with a larger max_nbr")
 
     # ---- host views ------------------------------------------------------------
+    def to_caller(self, x):
+        """Per-atom array in storage order -> the caller's atom order."""
+        return x[self.slot_of]
+
+    def from_caller(self, x):
+        """Per-atom array in the caller's order -> storage order."""
+        return x[self.order]
+
     def positions(self, dtype=torch.float64):
         return self.pos4[:, :3].to(dtype)
 

@@ -67,17 +67,14 @@ class EAMForceField:
             p.rho_hi[a] = float(np.float32(t.rho_min[a] + t.drho[a] * (t.n_rho - 1) * (1 - 1e-7)))
             p.inv_drho[a] = float(np.float32(t.inv_drho[a]))
         self.params = p
-        self.rho = None
-        self.fp = None
 
     def compute(self, eng, virial=True):
-        if self.rho is None or self.rho.numel() != eng.N or self.rho.device != eng.device:
-            self.rho = torch.zeros(eng.N, dtype=torch.float32, device=eng.device)
-            self.fp = torch.zeros(eng.N, dtype=torch.float32, device=eng.device)
+        rho = eng.scratch("eam_rho", torch.float32)
+        fp = eng.scratch("eam_fp", torch.float32)
         s = stream_ptr()
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(self.rho), ptr(self.fp),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp),
              ptr(eng.partial), s)
         call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(self.fp), ptr(eng.force),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(fp), ptr(eng.force),
              ptr(eng.partial), int(bool(virial)), s)

@@ -43,6 +43,7 @@ def test_pairs_exact_lj_cubic():
     eng = _lj_engine(pos, H)
     eng.setup()
     ei, d = eng.half_list()
+    pos = eng.positions().cpu().numpy()       # storage (Morton) order
     ref, dref = oracle_half_list(pos, H, 8.5, 1.0)
     assert pair_set(ei.cpu().numpy()) == pair_set(ref)
     assert np.array_equal(ei.cpu().numpy(), ref)
@@ -56,6 +57,7 @@ def test_pairs_exact_triclinic():
     eng = _lj_engine(pos, H)
     eng.setup()
     ei, _ = eng.half_list()
+    pos = eng.positions().cpu().numpy()
     ref, _ = oracle_half_list(pos, H, 8.5, 1.0)
     assert np.array_equal(ei.cpu().numpy(), ref)
 
@@ -66,6 +68,7 @@ def test_lj_forces_vs_fp64():
     eng = _lj_engine(pos, H)
     eng.setup()
     torch.cuda.synchronize()
+    pos = eng.positions().cpu().numpy()
     ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
     err = force_error(eng.force.cpu().numpy(), ref["forces"])
     th = eng.thermo_host()[0]
@@ -86,6 +89,7 @@ def test_lj_smoothing_modes(mode):
     eng = ReplicaEngine([spec], [lat.AR_MASS], cutoff=8.5, skin=1.0)
     eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, mode=mode)
     eng.setup()
+    pos = eng.positions().cpu().numpy()
     ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0,
                     mode=mode)
     assert force_error(eng.force.cpu().numpy(), ref["forces"]) <= FORCE_TOL
@@ -98,6 +102,7 @@ def test_lj_triclinic_forces():
     _print_case("lj_triclinic", n=len(pos))
     eng = _lj_engine(pos, H)
     eng.setup()
+    pos = eng.positions().cpu().numpy()
     ref = oracle_lj(pos, H, np.zeros(len(pos), int), [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
     err = force_error(eng.force.cpu().numpy(), ref["forces"])
     print(f"tri force err {err:.3e}", flush=True)
@@ -117,6 +122,7 @@ def test_eam_forces_vs_fp64(w_setfl):
     eng = ReplicaEngine([spec], [lat.W_MASS], cutoff=5.5, skin=1.0)
     eng.force_field = EAMForceField(w_setfl)
     eng.setup()
+    pos = eng.positions().cpu().numpy()
     ref = oracle_eam(pos, H, np.zeros(len(pos), int), w_setfl, 1.0)
     err = force_error(eng.force.cpu().numpy(), ref["forces"])
     th = eng.thermo_host()[0]
@@ -225,6 +231,14 @@ def test_nve_conservation_force_shift():
     E1 = th[0] + th[8]
     print(f"force-shift NVE 1 ps: E0 {E0:.6f} E1 {E1:.6f} (KE {th[8]:.3f})", flush=True)
     assert abs(E1 - E0) < 2e-3 * th[8]
+
+
+def test_caller_order_roundtrip():
+    pos, H = lat.w_bcc(5, seed=3, amp=0.05)
+    eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H)], [lat.W_MASS], cutoff=5.0, skin=1.0)
+    back = eng.to_caller(eng.positions()).cpu().numpy()
+    np.testing.assert_array_equal(back, pos)
+    assert not np.array_equal(eng.positions().cpu().numpy(), pos)   # storage order is Morton
 
 
 def test_graph_matches_eager():
