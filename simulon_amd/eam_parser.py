@@ -203,13 +203,17 @@ class EAMTables:
         nxt = torch.cat([t[..., 1:], t[..., -1:]], dim=-1)
         return torch.stack([t, nxt], dim=-1)
 
-    def device_arrays(self, device):
-        """Interleaved tables for the kernels (see pair_eam.hip)."""
-        p1 = torch.cat([self._pairs(self.density), self._pairs(self.pair)], dim=-1)
-        p2 = torch.cat([self._pairs(self.density_deriv), self._pairs(self.pair_deriv)], dim=-1)
+    def device_arrays(self, device, type_map=None):
+        """Interleaved tables for the kernels (see pair_eam.hip), re-indexed so
+        that row k belongs to engine atom type k = file element type_map[k]."""
+        tm = torch.arange(self.ne) if type_map is None else torch.as_tensor(type_map)
+        sel = lambda t: t[tm][:, tm]
+        p1 = torch.cat([self._pairs(sel(self.density)), self._pairs(sel(self.pair))], dim=-1)
+        p2 = torch.cat([self._pairs(sel(self.density_deriv)), self._pairs(sel(self.pair_deriv))],
+                       dim=-1)
         return {
-            "pass1": p1.contiguous().to(device),        # [ne, ne, n_r, 4]
+            "pass1": p1.contiguous().to(device),        # [nt, nt, n_r, 4]
             "pass2": p2.contiguous().to(device),
-            "emb": self._pairs(self.embed).contiguous().to(device),   # [ne, n_rho, 2]
-            "demb": self._pairs(self.embed_deriv).contiguous().to(device),
+            "emb": self._pairs(self.embed[tm]).contiguous().to(device),   # [nt, n_rho, 2]
+            "demb": self._pairs(self.embed_deriv[tm]).contiguous().to(device),
         }

@@ -408,6 +408,15 @@ class ReplicaEngine:
     def replica_state(self, r=0):
         return _lib.struct_from_tensor(self.reps, _lib.Replica, r)
 
+    def write_replica_state(self, r, P):
+        """Store a (modified) replica record; a captured graph keeps reading
+        the same device memory, so the change applies to later replays."""
+        import ctypes
+        sz = ctypes.sizeof(_lib.Replica)
+        raw = torch.frombuffer(bytearray(bytes(P)), dtype=torch.uint8)
+        self.reps[r * sz:(r + 1) * sz].copy_(raw.to(self.device))
+        self.specs[r].dt = P.dt
+
     def box(self, r=0):
         return np.array(self.replica_state(r).h[:], np.float64).reshape(3, 3)
 

@@ -46,26 +46,32 @@ class EAMForceField:
     """Two-pass EAM (core/force/eam_force_cu.py:461-579) from an EAM file."""
     name = "eam"
 
-    def __init__(self, source, n_r=8192, n_rho=4096, device="cuda"):
+    def __init__(self, source, n_r=8192, n_rho=4096, device="cuda", type_map=None):
+        """type_map[k]: index in the file's element list of engine atom type k
+        (default: identity)."""
         parser = source if isinstance(source, EAMParser) else EAMParser(source)
         self.parser = parser
         self.tables = EAMTables(parser, n_r, n_rho)
         self.cutoff = self.tables.cutoff
-        self.dev = self.tables.device_arrays(device)
+        tm = list(range(self.tables.ne)) if type_map is None else [int(k) for k in type_map]
+        if len(tm) > _lib.SIM_MAX_TYPES:
+            raise ValueError("at most 4 atom types")
+        self.type_map = tm
+        self.dev = self.tables.device_arrays(device, tm)
         t = self.tables
         p = _lib.EAMParams()
         p.pass1 = self.dev["pass1"].data_ptr()
         p.pass2 = self.dev["pass2"].data_ptr()
         p.emb = self.dev["emb"].data_ptr()
         p.demb = self.dev["demb"].data_ptr()
-        p.n_r, p.n_rho, p.ntypes = t.n_r, t.n_rho, t.ne
+        p.n_r, p.n_rho, p.ntypes = t.n_r, t.n_rho, len(tm)
         p.inv_dr = float(np.float32(t.inv_dr))
         p.r_clamp = float(np.float32(t.r_max * (1 - 1e-7)))
         p.cutoff = float(np.float32(t.cutoff))
-        for a in range(t.ne):
-            p.rho_lo[a] = float(np.float32(t.rho_min[a]))
-            p.rho_hi[a] = float(np.float32(t.rho_min[a] + t.drho[a] * (t.n_rho - 1) * (1 - 1e-7)))
-            p.inv_drho[a] = float(np.float32(t.inv_drho[a]))
+        for k, a in enumerate(tm):
+            p.rho_lo[k] = float(np.float32(t.rho_min[a]))
+            p.rho_hi[k] = float(np.float32(t.rho_min[a] + t.drho[a] * (t.n_rho - 1) * (1 - 1e-7)))
+            p.inv_drho[k] = float(np.float32(t.inv_drho[a]))
         self.params = p
 
     def compute(self, eng, virial=True):
