@@ -138,13 +138,34 @@ int sim_version(char* buf, int len);
  *   atom_cell, atom_slot: int[N] work arrays; cell_count/cell_start: int[C+1];
  *   cell_atoms: int[N]; overflow: int[1], set to 1 when a list exceeds max_nbr
  *   n: total atoms (the list's column stride); tiles: int2 (replica, first atom)
- *   precise: 1 = pair test in float64, 0 = in float32 (reference fp32 semantics) */
+ *   precise: 1 = pair test in float64, 0 = in float32 (reference fp32 semantics)
+ *   ro: optional physical reorder (NULL: atoms keep their slots). With ro, the
+ *       atoms of each rebuilding replica are permuted into cell order and
+ *       wrapped into the primary cell before the list is built, and the list is
+ *       also written as 16-bit slots of the 27-cell stencil (cell-tiled kernels;
+ *       requires >= 3 cells per dimension). */
+typedef struct sim_nl_reorder {
+    float* vel;            /* [N][3], permuted with the atoms */
+    float* force;          /* [N][3], permuted with the atoms */
+    long long* order;      /* [N] caller index of the atom in each slot */
+    int* slot_of;          /* [N] inverse of order */
+    int* image;            /* [N][4] lattice translation removed by wrapping */
+    double* pos_tmp;       /* work arrays of the same shapes */
+    float* vel_tmp;
+    float* force_tmp;
+    long long* order_tmp;
+    int* image_tmp;
+    int* cell_tmp;
+    unsigned short* nbr16; /* [N][max_nbr] row-major list of LDS slots, or NULL */
+    int* stencil_max;      /* largest 27-cell stencil population (atomic max), or NULL */
+} sim_nl_reorder_t;
+
 int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
-                 const double* pos4, double* pos_last, const int* flag,
+                 double* pos4, double* pos_last, const int* flag,
                  const int* cell_rep, int ncells,
                  int* atom_cell, int* atom_slot, int* cell_count, int* cell_start,
                  int* cell_atoms, int* nbr, int* nnbr, int max_nbr, int* overflow,
-                 int precise, void* stream);
+                 int precise, const sim_nl_reorder_t* ro, void* stream);
 
 /* Same build for a single system given as plain [N,3] coordinates, float32
  * (dtype 0) or float64 (dtype 1): the pair test uses exactly the reference's
@@ -193,6 +214,27 @@ int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
                   const double* pos4, const int* nbr, const int* nnbr, int n_stride,
                   sim_eam_params_t eam, const float* fp, float* force, double* partial,
                   int virial, void* stream);
+
+/* Cell-tiled variants (one workgroup per cell slot, ncells = total cell
+ * capacity): the 27-cell neighbourhood of the cell is staged in LDS (at most
+ * cap atoms, cap <= 2048; overflow bit 2 is set when a stencil holds more) and
+ * the lists are read as 16-bit slots (nbr16 from sim_nl_build with a reorder).
+ * Partials are written per cell slot: partial[ncells][16]. The EAM variants
+ * take one element type. */
+int sim_lj_force_cell(int ncells, const void* reps, const int* cell_rep, const int* cell_start,
+                      const int* cell_count, const double* pos4, const unsigned short* nbr16,
+                      const int* nnbr, int max_nbr, int cap, sim_lj_params_t lj, float* force,
+                      double* partial, int virial, int* overflow, void* stream);
+int sim_eam_density_cell(int ncells, const void* reps, const int* cell_rep,
+                         const int* cell_start, const int* cell_count, const double* pos4,
+                         const unsigned short* nbr16, const int* nnbr, int max_nbr, int cap,
+                         sim_eam_params_t eam, float* rho, float* fp, double* partial,
+                         int* overflow, void* stream);
+int sim_eam_force_cell(int ncells, const void* reps, const int* cell_rep, const int* cell_start,
+                       const int* cell_count, const double* pos4, const unsigned short* nbr16,
+                       const int* nnbr, int max_nbr, int cap, sim_eam_params_t eam,
+                       const float* fp, float* force, double* partial, int virial, int* overflow,
+                       void* stream);
 
 /* ---- integrator --------------------------------------------------------- */
 

@@ -57,12 +57,25 @@ class EAMParams(ctypes.Structure):
                 ("inv_drho", ctypes.c_float * SIM_MAX_TYPES)]
 
 
+class NLReorder(ctypes.Structure):
+    """Mirror of sim_nl_reorder_t."""
+    _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
+                                   "vel_tmp", "force_tmp", "order_tmp", "image_tmp", "cell_tmp",
+                                   "nbr16", "stencil_max")]
+
+
 # name -> (restype, argtypes)
 _I, _U64, _LL = ctypes.c_int, ctypes.c_ulonglong, ctypes.c_longlong
 SIGNATURES = {
     "sim_version": (_I, [ctypes.c_char_p, _I]),
     "sim_nl_build": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp,
-                          _vp, _vp, _I, _vp, _I, _vp]),
+                          _vp, _vp, _I, _vp, _I, _vp, _vp]),
+    "sim_lj_force_cell": (_I, [_I, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _I, _I, LJParams, _vp, _vp,
+                               _I, _vp, _vp]),
+    "sim_eam_density_cell": (_I, [_I, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _I, _I, EAMParams, _vp,
+                                  _vp, _vp, _vp, _vp]),
+    "sim_eam_force_cell": (_I, [_I, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _I, _I, EAMParams, _vp, _vp,
+                                _vp, _I, _vp, _vp]),
     "sim_nl_build_xyz": (_I, [_vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _vp, _I, _vp, _vp, _I, _vp,
                               _vp]),
     "sim_nl_half_count": (_I, [_vp, _vp, _I, _I, _vp, _vp, _vp]),

@@ -127,38 +127,39 @@ __global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2,
     maxd2[r] = 0ull;
 }
 
-// Per-replica compensated (Neumaier) sums of partial columns, one workgroup
-// per replica.
+// Per-replica sums of partial columns [c0, c1), one workgroup per replica:
+// each thread keeps a compensated (Neumaier) running sum per column over a
+// strided subset of the replica's tiles; the 256 thread sums are then added
+// in a fixed float64 tree (wave shuffles, then the 4 waves).
+template <int NC>
 __global__ void __launch_bounds__(256)
     k_reduce(const int* __restrict__ rep_tiles, const double* __restrict__ partial, int c0,
              int c1, double* __restrict__ thermo) {
-    __shared__ double ls[256], lc[256];
+    __shared__ double lds[4 * NC];
     const int r = blockIdx.x;
     const int b = rep_tiles[r], e = rep_tiles[r + 1];
-    for (int c = c0; c < c1; ++c) {
-        double s = 0.0, comp = 0.0;
-        for (int k = b + (int)threadIdx.x; k < e; k += 256) {
-            const double x = partial[(size_t)k * SIM_NPART + c];
-            const double tt = s + x;
-            comp += fabs(s) >= fabs(x) ? (s - tt) + x : (x - tt) + s;
-            s = tt;
+    double s[NC], comp[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = comp[c] = 0.0;
+    for (int k = b + (int)threadIdx.x; k < e; k += 256) {
+        const double* row = partial + (size_t)k * SIM_NPART;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (c0 + c >= c1) continue;
+            const double x = row[c0 + c];
+            const double tt = s[c] + x;
+            comp[c] += fabs(s[c]) >= fabs(x) ? (s[c] - tt) + x : (x - tt) + s[c];
+            s[c] = tt;
         }
-        ls[threadIdx.x] = s;
-        lc[threadIdx.x] = comp;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) {
-                const double x = ls[threadIdx.x + o];
-                const double a = ls[threadIdx.x];
-                const double tt = a + x;
-                const double cc = fabs(a) >= fabs(x) ? (a - tt) + x : (x - tt) + a;
-                ls[threadIdx.x] = tt;
-                lc[threadIdx.x] += lc[threadIdx.x + o] + cc;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) thermo[(size_t)r * SIM_NPART + c] = ls[0] + lc[0];
-        __syncthreads();
+    }
+    double v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = s[c] + comp[c];
+    block_sum<NC>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c0 + c < c1) thermo[(size_t)r * SIM_NPART + c0 + c] = v[c];
     }
 }
 
@@ -298,8 +299,12 @@ extern "C" int sim_reduce(int R, const int* rep_tiles, const double* partial, in
     if (R <= 0 || !rep_tiles || !partial || !thermo || col0 < 0 || col1 > SIM_NPART ||
         col0 >= col1)
         return SIM_E_ARG;
-    hipLaunchKernelGGL(k_reduce, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles, partial, col0,
-                       col1, thermo);
+    if (col1 - col0 <= 8)
+        hipLaunchKernelGGL(k_reduce<8>, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles, partial,
+                           col0, col1, thermo);
+    else
+        hipLaunchKernelGGL(k_reduce<SIM_NPART>, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles,
+                           partial, col0, col1, thermo);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

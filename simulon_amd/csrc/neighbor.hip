@@ -104,9 +104,20 @@ __global__ void k_bin(const sim_replica_t* reps, const SimTile* tiles, const T* 
 
 // Exclusive scan of cell_count[0..n) into cell_start[0..n], one workgroup of
 // 1024 threads, each owning a contiguous chunk.
-__global__ void __launch_bounds__(1024) k_scan(const int* cnt, int n, int* start) {
+__global__ void __launch_bounds__(1024) k_scan(const int* cnt, int n, int* start, const int* flag,
+                                               int R) {
     __shared__ int part[1024];
+    __shared__ int any;
     const int tid = threadIdx.x;
+    if (flag != nullptr) {
+        // nothing to do when no replica rebuilds: counts and starts are unchanged
+        if (tid == 0) any = 0;
+        __syncthreads();
+        for (int r = tid; r < R; r += 1024)
+            if (flag[r]) any = 1;
+        __syncthreads();
+        if (!any) return;
+    }
     const int chunk = (n + 1023) / 1024;
     const int b = tid * chunk, e = min(n, b + chunk);
     int s = 0;
@@ -155,6 +166,67 @@ __global__ void k_cell_order(const int* cell_rep, int ncells, const int* flag,
     }
 }
 
+// Physical reorder of every rebuilding replica into cell order, wrapping each
+// position into the primary cell (x -= floor(s) H) and accumulating the
+// lattice translation in `image`, so that the atoms of a cell are contiguous
+// and the cell-tiled pair kernels can stage whole cells with coalesced loads.
+__global__ void k_permute_gather(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
+                                 const int* cell_atoms, const int* atom_cell, const double4* pos,
+                                 const float* vel, const float* force, const long long* order,
+                                 const int4* image, double4* pos_t, float* vel_t, float* force_t,
+                                 long long* order_t, int4* image_t, int* cell_t) {
+    const SimTile t = tiles[blockIdx.x];
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    const int s = t.start + threadIdx.x;
+    if (s >= P.atom_base + P.natoms) return;
+    const int a = cell_atoms[s];
+    double4 x = pos[a];
+    const double* hi = P.hinv;
+    const double* h = P.h;
+    const double f0 = floor(x.x * hi[0] + x.y * hi[3] + x.z * hi[6]);
+    const double f1 = floor(x.x * hi[1] + x.y * hi[4] + x.z * hi[7]);
+    const double f2 = floor(x.x * hi[2] + x.y * hi[5] + x.z * hi[8]);
+    x.x -= f0 * h[0] + f1 * h[3] + f2 * h[6];
+    x.y -= f0 * h[1] + f1 * h[4] + f2 * h[7];
+    x.z -= f0 * h[2] + f1 * h[5] + f2 * h[8];
+    pos_t[s] = x;
+    int4 im = image[a];
+    im.x += (int)f0;
+    im.y += (int)f1;
+    im.z += (int)f2;
+    image_t[s] = im;
+    for (int k = 0; k < 3; ++k) {
+        vel_t[(size_t)s * 3 + k] = vel[(size_t)a * 3 + k];
+        force_t[(size_t)s * 3 + k] = force[(size_t)a * 3 + k];
+    }
+    order_t[s] = order[a];
+    cell_t[s] = atom_cell[a];
+}
+
+__global__ void k_permute_store(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
+                                const double4* pos_t, const float* vel_t, const float* force_t,
+                                const long long* order_t, const int4* image_t, const int* cell_t,
+                                double4* pos, float* vel, float* force, long long* order,
+                                int4* image, int* atom_cell, int* cell_atoms, int* slot_of) {
+    const SimTile t = tiles[blockIdx.x];
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    const int s = t.start + threadIdx.x;
+    if (s >= P.atom_base + P.natoms) return;
+    pos[s] = pos_t[s];
+    for (int k = 0; k < 3; ++k) {
+        vel[(size_t)s * 3 + k] = vel_t[(size_t)s * 3 + k];
+        force[(size_t)s * 3 + k] = force_t[(size_t)s * 3 + k];
+    }
+    const long long o = order_t[s];
+    order[s] = o;
+    image[s] = image_t[s];
+    atom_cell[s] = cell_t[s];
+    cell_atoms[s] = s;
+    slot_of[o] = s;
+}
+
 // Stencil offsets of one dimension: distinct cells only.
 __device__ __forceinline__ int stencil(int nc, int* off) {
     if (nc == 1) {
@@ -177,7 +249,7 @@ __global__ void __launch_bounds__(SIM_TILE)
     k_list(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
            const int* atom_cell, const int* cell_count, const int* cell_start,
            const int* cell_atoms, int* nbr, int* nnbr, int n_stride, int max_nbr,
-           int* overflow, double* pos_last) {
+           int* overflow, double* pos_last, unsigned short* nbr16, int* stencil_max) {
     const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
     if (!rep_active(flag, t.rep)) return;
     const sim_replica_t& P = reps[t.rep];
@@ -193,6 +265,7 @@ __global__ void __launch_bounds__(SIM_TILE)
     int ox[3], oy[3], oz[3];
     const int sx = stencil(nx, ox), sy = stencil(ny, oy), sz = stencil(nz, oz);
     int cnt = 0;
+    int soff = 0;   // running LDS offset of the current stencil cell (cell-tiled kernels)
     for (int a = 0; a < sz; ++a) {
         int zz = cz + oz[a];
         zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
@@ -213,15 +286,22 @@ __global__ void __launch_bounds__(SIM_TILE)
                     min_image_ref<TC>(dx, dy, dz, P);
                     const TC d2 = (dx * dx + dy * dy) + dz * dz;
                     if (d2 < rl2) {
-                        if (cnt < max_nbr) nbr[(size_t)cnt * n_stride + i] = j;
+                        if (cnt < max_nbr) {
+                            nbr[(size_t)cnt * n_stride + i] = j;
+                            if (nbr16 != nullptr)
+                                nbr16[(size_t)i * max_nbr + cnt] =
+                                    (unsigned short)(soff + (s - s0));
+                        }
                         ++cnt;
                     }
                 }
+                soff += s1 - s0;
             }
         }
     }
     nnbr[i] = cnt < max_nbr ? cnt : max_nbr;
     if (cnt > max_nbr) atomicOr(overflow, 1);
+    if (stencil_max != nullptr) atomicMax(stencil_max, soff);
     if (pos_last != nullptr) {
         pos_last[(size_t)i * 4 + 0] = (double)x[(size_t)i * STRIDE + 0];
         pos_last[(size_t)i * 4 + 1] = (double)x[(size_t)i * STRIDE + 1];
@@ -234,20 +314,36 @@ template <typename TC, typename T, int STRIDE>
 int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, const T* x,
                double* pos_last, const int* flag, const int* cell_rep, int ncells, int* atom_cell,
                int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms, int* nbr,
-               int* nnbr, int n_stride, int max_nbr, int* overflow, hipStream_t st) {
+               int* nnbr, int n_stride, int max_nbr, int* overflow, hipStream_t st,
+               const sim_nl_reorder_t* ro = nullptr) {
     hipLaunchKernelGGL(k_grid, dim3((R + 63) / 64), dim3(64), 0, st, R, reps, flag);
     hipLaunchKernelGGL(k_cell_clear, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
                        ncells, flag, cell_count);
     hipLaunchKernelGGL((k_bin<T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, x,
                        flag, cell_count, atom_cell, atom_slot);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, ncells, cell_start);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, ncells, cell_start, flag, R);
     hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                        atom_cell, atom_slot, cell_start, cell_atoms);
     hipLaunchKernelGGL(k_cell_order, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
                        ncells, flag, cell_count, cell_start, cell_atoms);
+    if (ro != nullptr && tiles != nullptr) {
+        auto* P4 = reinterpret_cast<double4*>(const_cast<T*>(x));
+        hipLaunchKernelGGL(k_permute_gather, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                           cell_atoms, atom_cell, P4, ro->vel, ro->force, ro->order,
+                           reinterpret_cast<const int4*>(ro->image),
+                           reinterpret_cast<double4*>(ro->pos_tmp), ro->vel_tmp, ro->force_tmp,
+                           ro->order_tmp, reinterpret_cast<int4*>(ro->image_tmp), ro->cell_tmp);
+        hipLaunchKernelGGL(k_permute_store, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                           reinterpret_cast<const double4*>(ro->pos_tmp), ro->vel_tmp,
+                           ro->force_tmp, ro->order_tmp,
+                           reinterpret_cast<const int4*>(ro->image_tmp), ro->cell_tmp, P4,
+                           ro->vel, ro->force, ro->order, reinterpret_cast<int4*>(ro->image),
+                           atom_cell, cell_atoms, ro->slot_of);
+    }
     hipLaunchKernelGGL((k_list<TC, T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles,
                        x, flag, atom_cell, cell_count, cell_start, cell_atoms, nbr, nnbr, n_stride,
-                       max_nbr, overflow, pos_last);
+                       max_nbr, overflow, pos_last, ro ? ro->nbr16 : nullptr,
+                       ro ? ro->stencil_max : nullptr);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
@@ -296,10 +392,11 @@ __global__ void k_half_fill(const sim_replica_t* rep, const T* x, int n, const i
 }  // namespace
 
 extern "C" int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
-                            const double* pos4, double* pos_last, const int* flag,
+                            double* pos4, double* pos_last, const int* flag,
                             const int* cell_rep, int ncells, int* atom_cell, int* atom_slot,
                             int* cell_count, int* cell_start, int* cell_atoms, int* nbr, int* nnbr,
-                            int max_nbr, int* overflow, int precise, void* stream) {
+                            int max_nbr, int* overflow, int precise, const sim_nl_reorder_t* ro,
+                            void* stream) {
     if (R <= 0 || !reps || !tiles || ntiles <= 0 || n <= 0 || !pos4 || !cell_rep || ncells <= 0 ||
         !atom_cell || !atom_slot || !cell_count || !cell_start || !cell_atoms || !nbr || !nnbr ||
         max_nbr <= 0 || !overflow)
@@ -310,10 +407,10 @@ extern "C" int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int
         return build_impl<double, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
                                              atom_cell, atom_slot, cell_count, cell_start,
                                              cell_atoms, nbr, nnbr, n, max_nbr, overflow,
-                                             as_stream(stream));
+                                             as_stream(stream), ro);
     return build_impl<float, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
                                         atom_cell, atom_slot, cell_count, cell_start, cell_atoms,
-                                        nbr, nnbr, n, max_nbr, overflow, as_stream(stream));
+                                        nbr, nnbr, n, max_nbr, overflow, as_stream(stream), ro);
 }
 
 extern "C" int sim_nl_build_xyz(void* rep, const void* xyz, int dtype, int n, int* atom_cell,
@@ -346,7 +443,7 @@ extern "C" int sim_nl_half_count(const int* nbr, const int* nnbr, int n, int max
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_half_count, dim3((n + 255) / 256), dim3(256), 0, st, nbr, nnbr, n,
                        half_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, half_count, n, half_start);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, half_count, n, half_start, nullptr, 0);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

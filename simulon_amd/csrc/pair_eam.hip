@@ -44,6 +44,9 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
     const int ti = (int)xi.w;
     const int nt = NT == 1 ? 1 : E.ntypes;
     int k = 0;
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * ns + i] : i;
     for (; k + U <= cnt; k += U) {
         int j[U];
         double4 xj[U];
@@ -51,7 +54,11 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
         Lerp L[U];
         float4 v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * ns + i];
+        for (int u = 0; u < U; ++u) j[u] = jn[u];
+        // indices of the next batch load while this batch computes
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * ns + i] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll
@@ -66,12 +73,13 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
             const int tj = NT == 1 ? 0 : (int)xj[u].w;
             v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
         }
+        // masked (not branched) so that the table loads of the batch stay
+        // unconditional and in flight together; + 0.0f leaves sums unchanged
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (r2[u] < thr) {
-                rho += lerp2(v[u].x, v[u].y, L[u].frac);
-                ep += lerp2(v[u].z, v[u].w, L[u].frac);
-            }
+            const float w = r2[u] < thr ? 1.0f : 0.0f;
+            rho += w * lerp2(v[u].x, v[u].y, L[u].frac);
+            ep += w * lerp2(v[u].z, v[u].w, L[u].frac);
         }
     }
     for (; k < cnt; ++k) {
@@ -166,6 +174,9 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
     const int ti = (int)xi.w;
     const int nt = NT == 1 ? 1 : E.ntypes;
     int k = 0;
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * ns + i] : i;
     for (; k + U <= cnt; k += U) {
         int j[U];
         double4 xj[U];
@@ -173,7 +184,10 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
         Lerp L[U];
         float4 va[U], vb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * ns + i];
+        for (int u = 0; u < U; ++u) j[u] = jn[u];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * ns + i] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll
@@ -197,12 +211,11 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (r2[u] < thr) {
-                const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
-                const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
-                const float s = fpi * dda + fpj[u] * ddb + lerp2(va[u].z, va[u].w, L[u].frac);
-                eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
-            }
+            const float w = r2[u] < thr ? 1.0f : 0.0f;
+            const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
+            const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
+            const float s = w * (fpi * dda + fpj[u] * ddb + lerp2(va[u].z, va[u].w, L[u].frac));
+            eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
         }
     }
     for (; k < cnt; ++k) {

@@ -79,11 +79,18 @@ __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const double4 xi, int c
     constexpr int U = SIM_NB_BATCH;
     const int ti = (int)xi.w;
     int k = 0;
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * n_stride + i] : i;
     for (; k + U <= cnt; k += U) {
         int j[U];
         double4 xj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) j[u] = nbr[(size_t)(k + u) * n_stride + i];
+        for (int u = 0; u < U; ++u) j[u] = jn[u];
+        // indices of the next batch load while this batch computes
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * n_stride + i] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll

@@ -25,6 +25,7 @@ Reference behaviour mirrored (JinZongxiao/Simulon):
   integrator constants      core/integrator/integrator.py:41-87
   Berendsen                 core/barostat.py:21-68
 """
+import ctypes
 import math
 from dataclasses import dataclass, field
 
@@ -107,7 +108,7 @@ class ReplicaEngine:
     """Packed multi-replica MD state and step on one GPU."""
 
     def __init__(self, specs, masses_amu, cutoff, skin, device="cuda", max_nbr=None,
-                 precise_list=True, seed=12345, velocity_seed=None, sort=True):
+                 precise_list=True, seed=12345, velocity_seed=None, sort=True, tiled=None):
         self.device = torch.device(device)
         if isinstance(specs, ReplicaSpec):
             specs = [specs]
@@ -124,6 +125,10 @@ class ReplicaEngine:
         _lib.load()
         self._layout()
         self._alloc(max_nbr, velocity_seed)
+        # cell-tiled pair kernels need >= 3 cells per dimension in every replica
+        can_tile = all(min(self._grid_of(r)) >= 3 for r in range(self.R))
+        self.tiled = can_tile if tiled is None else (bool(tiled) and can_tile)
+        self.cap = 0
         self.force_field = None
         self._scratch = {}
         self._graph = None
@@ -161,6 +166,26 @@ class ReplicaEngine:
         self.cell_cap = caps
         self.ncells = int(self.cell_base[-1])
         self.cell_rep_np = np.repeat(np.arange(R, dtype=np.int32), caps)
+
+    @property
+    def use_tiled(self):
+        return self.tiled and self.cap > 0
+
+    def pick_cap(self):
+        """LDS capacity of the cell-tiled kernels from the largest 27-cell
+        stencil population of the last build (host read, setup only)."""
+        if not self.tiled or self.cap:
+            return
+        smax = int(self.stencil_max.item())
+        need = int(smax * 1.3) + 32
+        self.cap = next((c for c in (512, 1024, 2048) if c >= need), 0)
+        if self.cap == 0:
+            self.tiled = False
+
+    def _grid_of(self, r):
+        w = perpendicular_widths(self.specs[r].box)
+        nc = np.maximum(1, np.floor(w / (self.rl * (1 + 1e-6)))).astype(np.int64)
+        return [int(min(x, 1024)) for x in nc]
 
     def _replica_struct(self, r):
         s = self.specs[r]
@@ -229,7 +254,7 @@ class ReplicaEngine:
         self.order = torch.from_numpy(order).to(dev)
         inv = np.empty_like(order)
         inv[order] = np.arange(N)
-        self.slot_of = torch.from_numpy(inv).to(dev)
+        self.slot_of = torch.from_numpy(inv.astype(np.int32)).to(dev)
         self.pos4 = torch.from_numpy(pos).to(dev)
         self.pos_last = self.pos4.clone()
         self.vel = torch.from_numpy(vel).to(dev)
@@ -260,6 +285,33 @@ class ReplicaEngine:
         self.partial = torch.zeros((self.ntiles, _lib.SIM_NPART), dtype=torch.float64, device=dev)
         self.thermo = torch.zeros((R, _lib.SIM_NPART), dtype=torch.float64, device=dev)
         self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+        # physical cell-order reorder at every rebuild (sim_nl_build with a reorder)
+        self.image = torch.zeros((N, 4), dtype=torch.int32, device=dev)
+        self._ro_tmp = {
+            "pos": torch.zeros((N, 4), dtype=torch.float64, device=dev),
+            "vel": torch.zeros((N, 3), dtype=torch.float32, device=dev),
+            "force": torch.zeros((N, 3), dtype=torch.float32, device=dev),
+            "order": torch.zeros(N, dtype=torch.int64, device=dev),
+            "image": torch.zeros((N, 4), dtype=torch.int32, device=dev),
+            "cell": torch.zeros(N, dtype=torch.int32, device=dev),
+        }
+        self.nbr16 = torch.zeros((N, self.max_nbr), dtype=torch.int16, device=dev)
+        self.stencil_max = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.partial_cell = torch.zeros((self.ncells, _lib.SIM_NPART), dtype=torch.float64,
+                                        device=dev)
+        self.rep_cells = torch.as_tensor(np.array(self.cell_base, np.int32), device=dev)
+        ro = _lib.NLReorder()
+        ro.vel, ro.force = self.vel.data_ptr(), self.force.data_ptr()
+        ro.order, ro.slot_of = self.order.data_ptr(), self.slot_of.data_ptr()
+        ro.image = self.image.data_ptr()
+        ro.pos_tmp, ro.vel_tmp = self._ro_tmp["pos"].data_ptr(), self._ro_tmp["vel"].data_ptr()
+        ro.force_tmp, ro.order_tmp = (self._ro_tmp["force"].data_ptr(),
+                                      self._ro_tmp["order"].data_ptr())
+        ro.image_tmp, ro.cell_tmp = (self._ro_tmp["image"].data_ptr(),
+                                     self._ro_tmp["cell"].data_ptr())
+        ro.nbr16 = self.nbr16.data_ptr()
+        ro.stencil_max = self.stencil_max.data_ptr()
+        self._ro = ro
         self.types_struct = _lib.Types()
         for t, m in enumerate(self.masses_amu):
             self.types_struct.mass[t] = float(internal_mass_f32(m))
@@ -287,7 +339,7 @@ class ReplicaEngine:
              ptr(self.pos4), ptr(self.pos_last), ptr(self.flag), ptr(self.cell_rep), self.ncells,
              ptr(self.atom_cell), ptr(self.atom_slot), ptr(self.cell_count), ptr(self.cell_start),
              ptr(self.cell_atoms), ptr(self.nbr), ptr(self.nnbr), self.max_nbr,
-             ptr(self.overflow), int(self.precise_list), s)
+             ptr(self.overflow), int(self.precise_list), ctypes.byref(self._ro), s)
 
     def compute_forces(self, virial=True):
         if self.force_field is None:
@@ -307,8 +359,22 @@ class ReplicaEngine:
         call("sim_step_advance", ptr(self.step_ctr), s)
 
     def reduce(self, col0=0, col1=15):
-        call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), col0, col1,
-             ptr(self.thermo), stream_ptr())
+        """float64 per-replica sums of the partial columns [col0, col1): force
+        columns (< T_KE) from the pair kernels' partials, kinetic columns from
+        the integrator's."""
+        s = stream_ptr()
+        fa, fb = col0, min(col1, _lib.T_KE)
+        if fa < fb:
+            if self.use_tiled:
+                call("sim_reduce", self.R, ptr(self.rep_cells), ptr(self.partial_cell), fa, fb,
+                     ptr(self.thermo), s)
+            else:
+                call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), fa, fb,
+                     ptr(self.thermo), s)
+        ka, kb = max(col0, _lib.T_KE), col1
+        if ka < kb:
+            call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
+                 ptr(self.thermo), s)
 
     def barostat(self):
         call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
@@ -322,6 +388,7 @@ class ReplicaEngine:
     def setup(self):
         """Initial list and forces at the current positions (md_model.py:91-102)."""
         self.build_list(force=True)
+        self.pick_cap()
         self.compute_forces(virial=True)
         self.reduce(0, 8)
         self._ke_now()
@@ -334,9 +401,7 @@ class ReplicaEngine:
         v = self.vel.double()
         prods = torch.stack([v[:, 0] * v[:, 0], v[:, 1] * v[:, 1], v[:, 2] * v[:, 2],
                              v[:, 0] * v[:, 1], v[:, 0] * v[:, 2], v[:, 1] * v[:, 2]], 1) * mi[:, None]
-        rep = torch.repeat_interleave(torch.arange(self.R, device=self.device),
-                                      torch.as_tensor(self.natoms, device=self.device))
-        K = torch.zeros((self.R, 6), dtype=torch.float64, device=self.device).index_add_(0, rep, prods)
+        K = torch.stack([prods[self.replica_slice(r)].sum(0) for r in range(self.R)])
         self.thermo[:, _lib.T_K:_lib.T_K + 6] = K
         self.thermo[:, _lib.T_KE] = 0.5 * (K[:, 0] + K[:, 1] + K[:, 2])
 
@@ -381,10 +446,15 @@ class ReplicaEngine:
             self.check()
 
     def check(self):
-        """Raise if a neighbor list overflowed (host read; call between chunks)."""
-        if int(self.overflow.item()) != 0:
+        """Raise if a neighbor list or an LDS stencil overflowed (host read; call
+        between chunks)."""
+        ov = int(self.overflow.item())
+        if ov & 1:
             raise RuntimeError(f"neighbor list overflow: an atom has more than {self.max_nbr} "
                                "neighbors; construct the engine with a larger max_nbr")
+        if ov & 2:
+            raise RuntimeError(f"cell stencil holds more than {self.cap} atoms (LDS capacity); "
+                               "construct the engine with tiled=False")
 
     # ---- host views ------------------------------------------------------------
     def to_caller(self, x):
@@ -395,12 +465,38 @@ class ReplicaEngine:
         """Per-atom array in the caller's order -> storage order."""
         return x[self.order]
 
-    def positions(self, dtype=torch.float64):
-        return self.pos4[:, :3].to(dtype)
+    def positions(self, dtype=torch.float64, unwrap=False):
+        """Positions in storage order. The rebuilds wrap atoms into the primary
+        cell (the forces are computed on these); unwrap=True adds back the
+        lattice translations, giving continuous trajectories."""
+        x = self.pos4[:, :3]
+        if unwrap:
+            H = torch.as_tensor(self.box_rows(), device=self.device)      # [R,3,3]
+            rep = self.atom_replica()
+            x = x + torch.einsum("nk,nkj->nj", self.image[:, :3].double(), H[rep])
+        return x.to(dtype)
+
+    def atom_replica(self):
+        if getattr(self, "_atom_rep", None) is None:
+            self._atom_rep = torch.repeat_interleave(
+                torch.arange(self.R, device=self.device),
+                torch.as_tensor(self.natoms, device=self.device))
+        return self._atom_rep
+
+    def box_rows(self):
+        """[R,3,3] current lattice vectors (rows) of every replica (device)."""
+        sz = ctypes.sizeof(_lib.Replica)
+        rec = self.reps.view(self.R, sz)[:, :72]
+        return rec.contiguous().view(torch.float64).view(self.R, 3, 3)
 
     def set_positions(self, xyz):
-        xyz = torch.as_tensor(xyz, device=self.device)
-        self.pos4[:, :3] = xyz.to(torch.float64)
+        """Replace the positions (storage order, continuous coordinates as
+        positions(unwrap=True) returns them). They are stored in the frame of
+        the last rebuild (minus the recorded lattice translations), so the
+        skin trigger keeps measuring true displacements."""
+        xyz = torch.as_tensor(xyz, device=self.device).to(torch.float64)
+        H = self.box_rows()[self.atom_replica()]
+        self.pos4[:, :3] = xyz - torch.einsum("nk,nkj->nj", self.image[:, :3].double(), H)
 
     def replica_slice(self, r):
         return slice(int(self.atom_base[r]), int(self.atom_base[r + 1]))

@@ -37,6 +37,12 @@ class LJForceField:
         self.rc = float(rc)
 
     def compute(self, eng, virial=True):
+        if eng.use_tiled:
+            call("sim_lj_force_cell", eng.ncells, ptr(eng.reps), ptr(eng.cell_rep),
+                 ptr(eng.cell_start), ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16),
+                 ptr(eng.nnbr), eng.max_nbr, eng.cap, self.params, ptr(eng.force),
+                 ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow), stream_ptr())
+            return
         call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(eng.force), ptr(eng.partial),
              int(bool(virial)), stream_ptr())
@@ -78,6 +84,15 @@ class EAMForceField:
         rho = eng.scratch("eam_rho", torch.float32)
         fp = eng.scratch("eam_fp", torch.float32)
         s = stream_ptr()
+        if eng.use_tiled and self.params.ntypes == 1:
+            common = (eng.ncells, ptr(eng.reps), ptr(eng.cell_rep), ptr(eng.cell_start),
+                      ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16), ptr(eng.nnbr),
+                      eng.max_nbr, eng.cap, self.params)
+            call("sim_eam_density_cell", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
+                 ptr(eng.overflow), s)
+            call("sim_eam_force_cell", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
+                 int(bool(virial)), ptr(eng.overflow), s)
+            return
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp),
              ptr(eng.partial), s)

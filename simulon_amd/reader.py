@@ -96,7 +96,7 @@ class AtomFileReader:
     # ---- state views (caller's atom order) -------------------------------------
     @property
     def coordinates(self):
-        return self.engine.to_caller(self.engine.positions(torch.float32))
+        return self.engine.to_caller(self.engine.positions(torch.float32, unwrap=True))
 
     @coordinates.setter
     def coordinates(self, x):
@@ -112,7 +112,10 @@ class AtomFileReader:
 
     @property
     def last_positions(self):
-        return self.engine.to_caller(self.engine.pos_last[:, :3].float())
+        eng = self.engine
+        H = eng.box_rows()[0]
+        x = eng.pos_last[:, :3] + eng.image[:, :3].double() @ H
+        return eng.to_caller(x.float())
 
     def _import_positions(self, x):
         x = torch.as_tensor(x, device=self.device)
@@ -150,6 +153,7 @@ class AtomFileReader:
 
     def update_neighbor_list(self):
         self._trigger_and_build(force=True)
+        self.engine.pick_cap()
         self.profile["effective_edges_last"] = int(self.get_cutoff_mask().sum().item())
 
     def _edges(self):
