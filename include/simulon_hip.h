@@ -158,6 +158,8 @@ typedef struct sim_nl_reorder {
     int* cell_tmp;
     unsigned short* nbr16; /* [N][max_nbr] row-major list of LDS slots, or NULL */
     int* stencil_max;      /* largest 27-cell stencil population (atomic max), or NULL */
+    float* vel_half;       /* [N][3] half-step velocities (a rebuild falls between the halves) */
+    float* vel_half_tmp;
 } sim_nl_reorder_t;
 
 int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,

@@ -61,7 +61,7 @@ class NLReorder(ctypes.Structure):
     """Mirror of sim_nl_reorder_t."""
     _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
                                    "vel_tmp", "force_tmp", "order_tmp", "image_tmp", "cell_tmp",
-                                   "nbr16", "stencil_max")]
+                                   "nbr16", "stencil_max", "vel_half", "vel_half_tmp")]
 
 
 # name -> (restype, argtypes)

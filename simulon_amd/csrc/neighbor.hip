@@ -174,7 +174,8 @@ __global__ void k_permute_gather(const sim_replica_t* reps, const SimTile* tiles
                                  const int* cell_atoms, const int* atom_cell, const double4* pos,
                                  const float* vel, const float* force, const long long* order,
                                  const int4* image, double4* pos_t, float* vel_t, float* force_t,
-                                 long long* order_t, int4* image_t, int* cell_t) {
+                                 long long* order_t, int4* image_t, int* cell_t,
+                                 const float* vh, float* vh_t) {
     const SimTile t = tiles[blockIdx.x];
     if (!rep_active(flag, t.rep)) return;
     const sim_replica_t& P = reps[t.rep];
@@ -199,6 +200,7 @@ __global__ void k_permute_gather(const sim_replica_t* reps, const SimTile* tiles
     for (int k = 0; k < 3; ++k) {
         vel_t[(size_t)s * 3 + k] = vel[(size_t)a * 3 + k];
         force_t[(size_t)s * 3 + k] = force[(size_t)a * 3 + k];
+        vh_t[(size_t)s * 3 + k] = vh[(size_t)a * 3 + k];
     }
     order_t[s] = order[a];
     cell_t[s] = atom_cell[a];
@@ -208,7 +210,8 @@ __global__ void k_permute_store(const sim_replica_t* reps, const SimTile* tiles,
                                 const double4* pos_t, const float* vel_t, const float* force_t,
                                 const long long* order_t, const int4* image_t, const int* cell_t,
                                 double4* pos, float* vel, float* force, long long* order,
-                                int4* image, int* atom_cell, int* cell_atoms, int* slot_of) {
+                                int4* image, int* atom_cell, int* cell_atoms, int* slot_of,
+                                const float* vh_t, float* vh) {
     const SimTile t = tiles[blockIdx.x];
     if (!rep_active(flag, t.rep)) return;
     const sim_replica_t& P = reps[t.rep];
@@ -218,6 +221,7 @@ __global__ void k_permute_store(const sim_replica_t* reps, const SimTile* tiles,
     for (int k = 0; k < 3; ++k) {
         vel[(size_t)s * 3 + k] = vel_t[(size_t)s * 3 + k];
         force[(size_t)s * 3 + k] = force_t[(size_t)s * 3 + k];
+        vh[(size_t)s * 3 + k] = vh_t[(size_t)s * 3 + k];
     }
     const long long o = order_t[s];
     order[s] = o;
@@ -327,18 +331,23 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
     hipLaunchKernelGGL(k_cell_order, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
                        ncells, flag, cell_count, cell_start, cell_atoms);
     if (ro != nullptr && tiles != nullptr) {
+        if (!ro->vel || !ro->force || !ro->order || !ro->slot_of || !ro->image || !ro->pos_tmp ||
+            !ro->vel_tmp || !ro->force_tmp || !ro->order_tmp || !ro->image_tmp || !ro->cell_tmp ||
+            !ro->vel_half || !ro->vel_half_tmp)
+            return SIM_E_ARG;
         auto* P4 = reinterpret_cast<double4*>(const_cast<T*>(x));
         hipLaunchKernelGGL(k_permute_gather, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                            cell_atoms, atom_cell, P4, ro->vel, ro->force, ro->order,
                            reinterpret_cast<const int4*>(ro->image),
                            reinterpret_cast<double4*>(ro->pos_tmp), ro->vel_tmp, ro->force_tmp,
-                           ro->order_tmp, reinterpret_cast<int4*>(ro->image_tmp), ro->cell_tmp);
+                           ro->order_tmp, reinterpret_cast<int4*>(ro->image_tmp), ro->cell_tmp,
+                           ro->vel_half, ro->vel_half_tmp);
         hipLaunchKernelGGL(k_permute_store, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                            reinterpret_cast<const double4*>(ro->pos_tmp), ro->vel_tmp,
                            ro->force_tmp, ro->order_tmp,
                            reinterpret_cast<const int4*>(ro->image_tmp), ro->cell_tmp, P4,
                            ro->vel, ro->force, ro->order, reinterpret_cast<int4*>(ro->image),
-                           atom_cell, cell_atoms, ro->slot_of);
+                           atom_cell, cell_atoms, ro->slot_of, ro->vel_half_tmp, ro->vel_half);
     }
     hipLaunchKernelGGL((k_list<TC, T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles,
                        x, flag, atom_cell, cell_count, cell_start, cell_atoms, nbr, nnbr, n_stride,

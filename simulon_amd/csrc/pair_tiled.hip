@@ -48,6 +48,10 @@ __device__ __forceinline__ bool load_halo(Halo<CAP>& H, const sim_replica_t& P, 
     const int nx = P.ncell[0], ny = P.ncell[1], nz = P.ncell[2];
     const int local = cell - P.cell_base;
     const int tid = threadIdx.x;
+    if (nx < 3 || ny < 3 || nz < 3) {   // the 27 stencil cells must be distinct
+        if (tid == 0) atomicOr(overflow, 4);
+        return false;
+    }
     if (tid < 27) {
         const int cx = local % nx, cy = (local / nx) % ny, cz = local / (nx * ny);
         const int oz = tid / 9 - 1, oy = (tid / 3) % 3 - 1, ox = tid % 3 - 1;

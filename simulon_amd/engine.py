@@ -294,6 +294,7 @@ class ReplicaEngine:
             "order": torch.zeros(N, dtype=torch.int64, device=dev),
             "image": torch.zeros((N, 4), dtype=torch.int32, device=dev),
             "cell": torch.zeros(N, dtype=torch.int32, device=dev),
+            "vel_half": torch.zeros((N, 3), dtype=torch.float32, device=dev),
         }
         self.nbr16 = torch.zeros((N, self.max_nbr), dtype=torch.int16, device=dev)
         self.stencil_max = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -311,6 +312,8 @@ class ReplicaEngine:
                                      self._ro_tmp["cell"].data_ptr())
         ro.nbr16 = self.nbr16.data_ptr()
         ro.stencil_max = self.stencil_max.data_ptr()
+        ro.vel_half, ro.vel_half_tmp = (self.vel_half.data_ptr(),
+                                        self._ro_tmp["vel_half"].data_ptr())
         self._ro = ro
         self.types_struct = _lib.Types()
         for t, m in enumerate(self.masses_amu):
@@ -454,6 +457,9 @@ class ReplicaEngine:
                                "neighbors; construct the engine with a larger max_nbr")
         if ov & 2:
             raise RuntimeError(f"cell stencil holds more than {self.cap} atoms (LDS capacity); "
+                               "construct the engine with tiled=False")
+        if ov & 4:
+            raise RuntimeError("a replica's cell grid fell below 3 cells per dimension; "
                                "construct the engine with tiled=False")
 
     # ---- host views ------------------------------------------------------------
