@@ -450,7 +450,8 @@ extern "C" int sim_eam_density_cell(int ncells, const void* reps, const int* cel
     hipLaunchKernelGGL((k_eam_density_cell<C, 8>), dim3(ncells), dim3(SIM_CELL_THREADS), 0, st, \
                        P, cell_rep, cell_start, cell_count, X, nbr16, nnbr, max_nbr, eam, rho, \
                        fp, partial, overflow, thr)
-    SIM_CAP_DISPATCH(512, L_DENS) SIM_CAP_DISPATCH(1024, L_DENS) SIM_CAP_DISPATCH(2048, L_DENS) {}
+    SIM_CAP_DISPATCH(512, L_DENS) SIM_CAP_DISPATCH(1024, L_DENS) SIM_CAP_DISPATCH(1280, L_DENS)
+    SIM_CAP_DISPATCH(1536, L_DENS) SIM_CAP_DISPATCH(2048, L_DENS) {}
 #undef L_DENS
     SIM_CHECK_LAUNCH();
     return SIM_OK;
@@ -478,9 +479,11 @@ extern "C" int sim_eam_force_cell(int ncells, const void* reps, const int* cell_
                        st, P, cell_rep, cell_start, cell_count, X, nbr16, nnbr, max_nbr, eam, fp,  \
                        force, partial, overflow, thr)
     if (virial) {
-        SIM_CAP_DISPATCH(512, L_FV) SIM_CAP_DISPATCH(1024, L_FV) SIM_CAP_DISPATCH(2048, L_FV) {}
+        SIM_CAP_DISPATCH(512, L_FV) SIM_CAP_DISPATCH(1024, L_FV) SIM_CAP_DISPATCH(1280, L_FV)
+        SIM_CAP_DISPATCH(1536, L_FV) SIM_CAP_DISPATCH(2048, L_FV) {}
     } else {
-        SIM_CAP_DISPATCH(512, L_F) SIM_CAP_DISPATCH(1024, L_F) SIM_CAP_DISPATCH(2048, L_F) {}
+        SIM_CAP_DISPATCH(512, L_F) SIM_CAP_DISPATCH(1024, L_F) SIM_CAP_DISPATCH(1280, L_F)
+        SIM_CAP_DISPATCH(1536, L_F) SIM_CAP_DISPATCH(2048, L_F) {}
     }
 #undef L_FV
 #undef L_F
@@ -510,9 +513,11 @@ extern "C" int sim_lj_force_cell(int ncells, const void* reps, const int* cell_r
                        cell_rep, cell_start, cell_count, X, nbr16, nnbr, max_nbr, lj, force,     \
                        partial, overflow, thr)
     if (virial) {
-        SIM_CAP_DISPATCH(512, L_LV) SIM_CAP_DISPATCH(1024, L_LV) SIM_CAP_DISPATCH(2048, L_LV) {}
+        SIM_CAP_DISPATCH(512, L_LV) SIM_CAP_DISPATCH(1024, L_LV) SIM_CAP_DISPATCH(1280, L_LV)
+        SIM_CAP_DISPATCH(1536, L_LV) SIM_CAP_DISPATCH(2048, L_LV) {}
     } else {
-        SIM_CAP_DISPATCH(512, L_L) SIM_CAP_DISPATCH(1024, L_L) SIM_CAP_DISPATCH(2048, L_L) {}
+        SIM_CAP_DISPATCH(512, L_L) SIM_CAP_DISPATCH(1024, L_L) SIM_CAP_DISPATCH(1280, L_L)
+        SIM_CAP_DISPATCH(1536, L_L) SIM_CAP_DISPATCH(2048, L_L) {}
     }
 #undef L_LV
 #undef L_L

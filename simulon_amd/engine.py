@@ -127,8 +127,12 @@ class ReplicaEngine:
         self._alloc(max_nbr, velocity_seed)
         # cell-tiled pair kernels need >= 3 cells per dimension in every replica
         can_tile = all(min(self._grid_of(r)) >= 3 for r in range(self.R))
+        # None = automatic: tiled when the stencils fit the 1024-atom LDS tile
+        self._tiled_auto = tiled is None
         self.tiled = can_tile if tiled is None else (bool(tiled) and can_tile)
         self.cap = 0
+        # stencil populations only change at rebuilds; overflow is checked
+        self.cap_margin = 1.08
         self.force_field = None
         self._scratch = {}
         self._graph = None
@@ -177,8 +181,9 @@ class ReplicaEngine:
         if not self.tiled or self.cap:
             return
         smax = int(self.stencil_max.item())
-        need = int(smax * 1.3) + 32
-        self.cap = next((c for c in (512, 1024, 2048) if c >= need), 0)
+        need = int(smax * self.cap_margin) + 16
+        caps = (512, 1024) if self._tiled_auto else (512, 1024, 1280, 1536, 2048)
+        self.cap = next((c for c in caps if c >= need), 0)
         if self.cap == 0:
             self.tiled = False
 
