@@ -262,9 +262,10 @@ int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
 int sim_step_advance(unsigned long long* step_dev, void* stream);
 
 /* Per-replica compensated FP64 sums of partial columns [col0, col1) over the
- * replica's tiles (rep_tiles: int[R+1]) into thermo[R][16]. */
-int sim_reduce(int R, const int* rep_tiles, const double* partial, int col0, int col1,
-               double* thermo, void* stream);
+ * replica's rows of `partial` (rep_rows: int[R+1], tiles or cell slots) into
+ * thermo[R][16]; chunks: double[R][64][16] work array. */
+int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int col1,
+               double* chunks, double* thermo, void* stream);
 
 /* Berendsen (barostat.py:36-68) for replicas with barostat = 1: pressure from
  * thermo K + W and the volume, mu, box update in reps; then x, v and

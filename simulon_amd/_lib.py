@@ -89,7 +89,7 @@ SIGNATURES = {
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,
                                   _vp, _vp]),
     "sim_step_advance": (_I, [_vp, _vp]),
-    "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp]),
+    "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp]),
     "sim_berendsen": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp]),
     "sim_pos_export": (_I, [_vp, _I, _vp, _I, _vp]),
     "sim_pos_import": (_I, [_vp, _I, _vp, _I, _vp]),

@@ -127,17 +127,22 @@ __global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2,
     maxd2[r] = 0ull;
 }
 
-// Per-replica sums of partial columns [c0, c1), one workgroup per replica:
-// each thread keeps a compensated (Neumaier) running sum per column over a
-// strided subset of the replica's tiles; the 256 thread sums are then added
-// in a fixed float64 tree (wave shuffles, then the 4 waves).
+// Per-replica sums of partial columns [c0, c1) in two levels: workgroup
+// (r, k) sums chunk k of replica r's rows with a compensated (Neumaier)
+// running sum per column and a fixed float64 tree across its 256 threads;
+// one workgroup per replica then adds the SIM_RED_CHUNKS chunk sums in order.
+// Both levels have a fixed order, so the result is deterministic.
+#define SIM_RED_CHUNKS 64
+
 template <int NC>
 __global__ void __launch_bounds__(256)
-    k_reduce(const int* __restrict__ rep_tiles, const double* __restrict__ partial, int c0,
-             int c1, double* __restrict__ thermo) {
+    k_reduce_chunks(const int* __restrict__ rep_rows, const double* __restrict__ partial, int c0,
+                    int c1, double* __restrict__ chunks) {
     __shared__ double lds[4 * NC];
-    const int r = blockIdx.x;
-    const int b = rep_tiles[r], e = rep_tiles[r + 1];
+    const int r = blockIdx.y, ck = blockIdx.x;
+    const int b0 = rep_rows[r], e0 = rep_rows[r + 1];
+    const int len = (e0 - b0 + SIM_RED_CHUNKS - 1) / SIM_RED_CHUNKS;
+    const int b = b0 + ck * len, e = min(e0, b + len);
     double s[NC], comp[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) s[c] = comp[c] = 0.0;
@@ -145,8 +150,7 @@ __global__ void __launch_bounds__(256)
         const double* row = partial + (size_t)k * SIM_NPART;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            if (c0 + c >= c1) continue;
-            const double x = row[c0 + c];
+            const double x = c0 + c < c1 ? row[c0 + c] : 0.0;
             const double tt = s[c] + x;
             comp[c] += fabs(s[c]) >= fabs(x) ? (s[c] - tt) + x : (x - tt) + s[c];
             s[c] = tt;
@@ -157,10 +161,25 @@ __global__ void __launch_bounds__(256)
     for (int c = 0; c < NC; ++c) v[c] = s[c] + comp[c];
     block_sum<NC>(v, lds);
     if (threadIdx.x == 0) {
+        double* out = chunks + ((size_t)r * SIM_RED_CHUNKS + ck) * SIM_NPART;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-            if (c0 + c < c1) thermo[(size_t)r * SIM_NPART + c0 + c] = v[c];
+            if (c0 + c < c1) out[c0 + c] = v[c];
     }
+}
+
+__global__ void k_reduce_final(int c0, int c1, const double* __restrict__ chunks,
+                               double* __restrict__ thermo) {
+    const int r = blockIdx.x, c = c0 + (int)threadIdx.x;
+    if (c >= c1) return;
+    double s = 0.0, comp = 0.0;
+    for (int k = 0; k < SIM_RED_CHUNKS; ++k) {
+        const double x = chunks[((size_t)r * SIM_RED_CHUNKS + k) * SIM_NPART + c];
+        const double tt = s + x;
+        comp += fabs(s) >= fabs(x) ? (s - tt) + x : (x - tt) + s;
+        s = tt;
+    }
+    thermo[(size_t)r * SIM_NPART + c] = s + comp;
 }
 
 // Berendsen, core/barostat.py:46-66 (pressure from K + W, mu^3 clamp, box and
@@ -294,17 +313,20 @@ extern "C" int sim_nl_trigger(int R, void* reps, double* maxd2, int* flag, int* 
     return SIM_OK;
 }
 
-extern "C" int sim_reduce(int R, const int* rep_tiles, const double* partial, int col0, int col1,
-                          double* thermo, void* stream) {
-    if (R <= 0 || !rep_tiles || !partial || !thermo || col0 < 0 || col1 > SIM_NPART ||
+extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int col1,
+                          double* chunks, double* thermo, void* stream) {
+    if (R <= 0 || !rep_rows || !partial || !chunks || !thermo || col0 < 0 || col1 > SIM_NPART ||
         col0 >= col1)
         return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    const dim3 g(SIM_RED_CHUNKS, R);
     if (col1 - col0 <= 8)
-        hipLaunchKernelGGL(k_reduce<8>, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles, partial,
-                           col0, col1, thermo);
+        hipLaunchKernelGGL(k_reduce_chunks<8>, g, dim3(256), 0, st, rep_rows, partial, col0, col1,
+                           chunks);
     else
-        hipLaunchKernelGGL(k_reduce<SIM_NPART>, dim3(R), dim3(256), 0, as_stream(stream), rep_tiles,
-                           partial, col0, col1, thermo);
+        hipLaunchKernelGGL(k_reduce_chunks<SIM_NPART>, g, dim3(256), 0, st, rep_rows, partial,
+                           col0, col1, chunks);
+    hipLaunchKernelGGL(k_reduce_final, dim3(R), dim3(SIM_NPART), 0, st, col0, col1, chunks, thermo);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

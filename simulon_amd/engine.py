@@ -289,6 +289,7 @@ class ReplicaEngine:
         self.maxd2 = torch.zeros(R, dtype=torch.float64, device=dev)
         self.partial = torch.zeros((self.ntiles, _lib.SIM_NPART), dtype=torch.float64, device=dev)
         self.thermo = torch.zeros((R, _lib.SIM_NPART), dtype=torch.float64, device=dev)
+        self.red_chunks = torch.zeros((R, 64, _lib.SIM_NPART), dtype=torch.float64, device=dev)
         self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
         # physical cell-order reorder at every rebuild (sim_nl_build with a reorder)
         self.image = torch.zeros((N, 4), dtype=torch.int32, device=dev)
@@ -375,14 +376,14 @@ class ReplicaEngine:
         if fa < fb:
             if self.use_tiled:
                 call("sim_reduce", self.R, ptr(self.rep_cells), ptr(self.partial_cell), fa, fb,
-                     ptr(self.thermo), s)
+                     ptr(self.red_chunks), ptr(self.thermo), s)
             else:
                 call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), fa, fb,
-                     ptr(self.thermo), s)
+                     ptr(self.red_chunks), ptr(self.thermo), s)
         ka, kb = max(col0, _lib.T_KE), col1
         if ka < kb:
             call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
-                 ptr(self.thermo), s)
+                 ptr(self.red_chunks), ptr(self.thermo), s)
 
     def barostat(self):
         call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,

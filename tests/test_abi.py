@@ -40,7 +40,7 @@ def test_argument_errors_without_gpu():
     """Argument checks run before any launch and return SIM_E_ARG (-1)."""
     from simulon_amd import _lib
     lib = _lib.load()
-    assert lib.sim_reduce(0, None, None, 0, 1, None, None) == -1
+    assert lib.sim_reduce(0, None, None, 0, 1, None, None, None) == -1
     assert lib.sim_nl_trigger(1, None, None, None, None, 0, None) == -1
     assert lib.sim_pos_export(None, 10, None, 0, None) == -1
 
