@@ -115,7 +115,7 @@ def eam_kernel_roofline(eng):
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
-            "other_pass_ms": round(t1 if name == "eam_force" else t2, 5)}
+            "other_pass_ms": round(t1 if t == t2 else t2, 5)}
 
 
 def lj_kernel_roofline(eng):

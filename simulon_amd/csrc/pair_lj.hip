@@ -27,7 +27,7 @@ __device__ __forceinline__ void lj_pair(LJAcc& a, float rx, float ry, float rz, 
                                         float eps, float sig, const sim_lj_params_t& lj) {
     if (!(r2 < thr)) return;
     const float rc = lj.rc;
-    const float inv_r2 = 1.0f / r2;
+    const float inv_r2 = __builtin_amdgcn_rcpf(r2);   // 1 ulp
     const float s2 = sig * sig * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;

@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(SIM_CELL_THREADS)
                     const int tj = (int)H.aux[s];
                     const float eps = lj.eps[ti * SIM_MAX_TYPES + tj];
                     const float sig = lj.sig[ti * SIM_MAX_TYPES + tj];
-                    const float inv_r2 = 1.0f / r2;
+                    const float inv_r2 = __builtin_amdgcn_rcpf(r2);   // 1 ulp
                     const float s2 = sig * sig * inv_r2;
                     const float s6 = s2 * s2 * s2;
                     const float s12 = s6 * s6;
