@@ -37,6 +37,9 @@
  *        force fields (e.g. core/force/lennard_jones_force.py:92-94),
  *        core/md_model.py:83-86 and integrator.py:118-119
  *   sim_berendsen                  core/barostat.py:36-68
+ *   sim_mlp_*                      the learned-potential backbone slot of
+ *        machine_learning_potentials/machine_learning_force.py:85-142 (the
+ *        network itself is this library's own, see the MLP section below)
  */
 #ifndef SIMULON_HIP_H
 #define SIMULON_HIP_H
@@ -277,6 +280,80 @@ int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double*
  * array, and back (sim_pos_import). */
 int sim_pos_export(const double* pos4, int n, void* xyz, int dtype, void* stream);
 int sim_pos_import(double* pos4, int n, const void* xyz, int dtype, void* stream);
+
+/* ---- learned radial-descriptor MLP potential ----------------------------
+ *
+ * The reference's learned potential is an external model behind the return
+ * contract of machine_learning_potentials/machine_learning_force.py:85-142
+ * ({'energy', 'forces', 'virial'}); these entry points implement this
+ * library's own network under the same contract:
+ *   g_ik = sum_{j, r_ij < rc} exp(-(r_ij - mu_k)^2 / (2 w^2)) fc(r_ij),
+ *          mu_k = mu0 + k dmu (k < 32), fc(r) = (cos(pi r / rc) + 1) / 2
+ *   E_i  = w3 . silu(W2 silu(W1 (g_i - mean) inv_std + b1) + b2) + b3
+ * W1 [64][32] and W2 [64][64] are bfloat16 (MFMA operands, float32
+ * accumulation); biases, w3, mean and inv_std are float32.
+ * Partial rows: SIM_MLP_ROWS_PER_TILE rows per tile (partial[ntiles*4][16]);
+ * sim_reduce over rep_rows = 4 * tile offsets gives the replica totals. */
+#define SIM_MLP_NDESC 32
+#define SIM_MLP_NH 64
+#define SIM_MLP_ROWS_PER_TILE 4
+/* flat parameter / gradient vector: W1 (2048) b1 (64) W2 (4096) b2 (64) w3 (64) b3 (1) */
+#define SIM_MLP_NPARAM 6337
+#define SIM_MLP_NGRAD (SIM_MLP_NPARAM + 1) /* gradient, then the loss value */
+
+typedef struct sim_mlp_net {
+    void* w1;             /* bf16 [64][32] row-major (hidden unit, descriptor) */
+    void* w2;             /* bf16 [64][64] */
+    float* b1;            /* [64] */
+    float* b2;            /* [64] */
+    float* w3;            /* [64] */
+    float* b3;            /* [1] */
+    float* mean;          /* [32] descriptor shift */
+    float* inv_std;       /* [32] descriptor scale */
+    float rc;             /* cutoff, Angstrom */
+    float mu0;            /* first Gaussian centre */
+    float dmu;            /* centre spacing */
+    float inv_w2;         /* 1 / w^2 */
+} sim_mlp_net_t;
+
+/* Descriptors and the network for every atom: per-atom dE_i/dg_i (c, float
+ * [N][32], already divided by the descriptor scale), optional per-atom
+ * energies e_atom [N], optional normalised descriptors gh [N][32], and the
+ * energy column of partial. */
+int sim_mlp_energy(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+                   const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net, float* c,
+                   float* gh, float* e_atom, double* partial, void* stream);
+
+/* Analytic forces F_i = -sum_j (c_i + c_j) . dphi(r_ij)/dr r_ij/|r_ij| and the
+ * virial columns of partial (same convention as sim_eam_force). */
+int sim_mlp_force(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+                  const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net, const float* c,
+                  float* force, double* partial, int virial, void* stream);
+
+/* Force-matching adjoint: with R_i = 2 wF (F_i - Fref_i) / (3 N_replica),
+ * v_i = inv_std * dL/dc_i = -inv_std * sum_j dphi(r_ij)/dr ((R_i - R_j) . r_ij/|r_ij|). */
+int sim_mlp_adjoint(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+                    const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net,
+                    const float* force, const float* force_ref, float wF, float* v, void* stream);
+
+/* Weight gradient of the loss
+ *   L = sum_r [ wE (E_r - Eref_r)^2 / N_r^2 + wF sum_{i in r} |F_i - Fref_i|^2 / (3 N_r) ]
+ * from gh (sim_mlp_energy), v (sim_mlp_adjoint), the replica energies
+ * thermo[r][0] (sim_reduce of the energy column) and the forces: a
+ * forward-over-reverse pass through the network per atom, BF16-MFMA outer
+ * products over 16-atom groups, per-workgroup partials (grad_partial, float
+ * [max_blocks][SIM_MLP_NGRAD]) and a fixed-order float64 sum into grad
+ * [SIM_MLP_NGRAD] (the last entry is L).  atom_rep: int[n] replica of each atom. */
+int sim_mlp_grad(int n, const int* atom_rep, const void* reps, const double* thermo,
+                 const double* e_ref, float wE, float wF, sim_mlp_net_t net, const float* gh,
+                 const float* v, const float* force, const float* force_ref, float* grad_partial,
+                 int max_blocks, float* grad, void* stream);
+
+/* Adam step on the float32 master parameters (flat layout above), then the
+ * network arrays of net are rewritten from them (W1, W2 rounded to bfloat16,
+ * nearest-even).  t: 1-based step number. */
+int sim_mlp_adam(float* master, const float* grad, float* m, float* v, int t, float lr,
+                 float beta1, float beta2, float eps, sim_mlp_net_t net, void* stream);
 
 #ifdef __cplusplus
 }
