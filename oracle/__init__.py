@@ -18,7 +18,7 @@ PyTorch CPU path for one MD step:
   barostat.py    core/barostat.py:36-68              Berendsen
   mlp.py         radial-descriptor MLP potential (no reference counterpart:
                  the reference's learned potential is an external CHGNet
-                 wrapper; this restates the build's own network in FP64)
+                 wrapper; this restates the library's own network in FP64)
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this package, and only as the checker (or the timed CPU baseline).

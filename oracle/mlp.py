@@ -2,7 +2,7 @@
 
 There is no reference counterpart: the reference's learned potential is an
 external CHGNet wrapper (machine_learning_potentials/machine_learning_force.py,
-return contract :85-142). This module restates the build's own network with
+return contract :85-142). This module restates the library's own network with
 torch autograd on the CPU in float64, so that the device kernels' energies,
 forces and weight gradients can be checked against it.
 
@@ -18,9 +18,10 @@ Weights may be given already rounded to bfloat16 (the device stores them so).
 Loss for one frame (force matching, as trained on the device):
   L = wE (E - E_ref)^2 / N^2 + wF sum_i |F_i - F_ref_i|^2 / (3 N)
 
-pinned by: F = -dE/dx by central differences, the descriptor closed form for
-two atoms, and the loss gradient against central differences of the loss
-(tests/test_oracle_mlp.py).
+pinned by: F = -dE/dx by central differences, the descriptor of a perfect bcc
+lattice from its shell coordination numbers, invariance under rigid motions,
+the virial against a strain derivative, and the loss gradient against central
+differences of the loss (tests/test_oracle_mlp.py).
 """
 import math
 
@@ -92,7 +93,35 @@ def energy_forces(pos, H, params, mean, std, create_graph=False):
     e = network(params, gh)
     E = e.sum()
     (dEdx,) = torch.autograd.grad(E, x, create_graph=create_graph)
+    if not create_graph:
+        return E.detach(), -dEdx, e.detach(), g.detach()
     return E, -dEdx, e, g
+
+
+def energy_virial(pos, H, params, mean, std):
+    """(E, W [3,3], scalar virial) with W = -sum_pairs d (x) dE/dd over the
+    minimum-image pair vectors d = x_i - x_j, and the scalar sum_pairs d . dE/dd.
+
+    pinned by: -W equals the derivative of E under a homogeneous strain of
+    positions and box (tests/test_oracle_mlp.py)."""
+    pos = np.asarray(pos, np.float64)
+    i, j = pairs(pos, H)
+    Ht = torch.as_tensor(np.asarray(H, np.float64))
+    x = torch.as_tensor(pos)
+    d = x[i] - x[j]
+    s = d @ torch.linalg.inv(Ht)
+    d = ((s - torch.round(s)) @ Ht).clone().requires_grad_(True)
+    r = torch.sqrt((d * d).sum(-1))
+    mu = torch.as_tensor(MU)
+    fc = 0.5 * (torch.cos(math.pi * r / RC) + 1.0) * (r < RC)
+    phi = torch.exp(-((r[:, None] - mu[None, :]) ** 2) / (2 * WIDTH ** 2)) * fc[:, None]
+    g = torch.zeros((len(pos), NDESC), dtype=torch.float64)
+    g = g.index_add(0, torch.as_tensor(i), phi).index_add(0, torch.as_tensor(j), phi)
+    gh = (g - torch.as_tensor(mean)) / torch.as_tensor(std)
+    E = network(params, gh).sum()
+    (dEdd,) = torch.autograd.grad(E, d)
+    W = -(d.detach()[:, :, None] * dEdd[:, None, :]).sum(0)
+    return float(E.detach()), W.numpy(), float((d.detach() * dEdd).sum())
 
 
 def loss(frames, params, mean, std, wE=1.0, wF=1.0):

@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsimulon_
 SIM_TILE = 256
 SIM_MAX_TYPES = 4
 SIM_NPART = 16
+MLP_NDESC, MLP_NH, MLP_ROWS_PER_TILE = 32, 64, 4
+MLP_NPARAM = 6337
+MLP_NGRAD = MLP_NPARAM + 1
 T_EPOT, T_VIRIAL, T_W, T_KE, T_K = 0, 1, 2, 8, 9
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -57,6 +60,13 @@ class EAMParams(ctypes.Structure):
                 ("inv_drho", ctypes.c_float * SIM_MAX_TYPES)]
 
 
+class MLPNet(ctypes.Structure):
+    """Mirror of sim_mlp_net_t."""
+    _fields_ = [(n, _vp) for n in ("w1", "w2", "b1", "b2", "w3", "b3", "mean", "inv_std")] + [
+        ("rc", ctypes.c_float), ("mu0", ctypes.c_float), ("dmu", ctypes.c_float),
+        ("inv_w2", ctypes.c_float)]
+
+
 class NLReorder(ctypes.Structure):
     """Mirror of sim_nl_reorder_t."""
     _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
@@ -93,6 +103,14 @@ SIGNATURES = {
     "sim_berendsen": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp]),
     "sim_pos_export": (_I, [_vp, _I, _vp, _I, _vp]),
     "sim_pos_import": (_I, [_vp, _I, _vp, _I, _vp]),
+    "sim_mlp_energy": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, MLPNet, _vp, _vp, _vp, _vp, _vp]),
+    "sim_mlp_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, MLPNet, _vp, _vp, _vp, _I, _vp]),
+    "sim_mlp_adjoint": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, MLPNet, _vp, _vp, ctypes.c_float,
+                             _vp, _vp]),
+    "sim_mlp_grad": (_I, [_I, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, MLPNet, _vp, _vp,
+                          _vp, _vp, _vp, _I, _vp, _vp]),
+    "sim_mlp_adam": (_I, [_vp, _vp, _vp, _vp, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, MLPNet, _vp]),
 }
 
 _lib = None

@@ -374,16 +374,22 @@ class ReplicaEngine:
         s = stream_ptr()
         fa, fb = col0, min(col1, _lib.T_KE)
         if fa < fb:
-            if self.use_tiled:
-                call("sim_reduce", self.R, ptr(self.rep_cells), ptr(self.partial_cell), fa, fb,
-                     ptr(self.red_chunks), ptr(self.thermo), s)
-            else:
-                call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), fa, fb,
-                     ptr(self.red_chunks), ptr(self.thermo), s)
+            part, rows = self.force_partials()
+            call("sim_reduce", self.R, ptr(rows), ptr(part), fa, fb, ptr(self.red_chunks),
+                 ptr(self.thermo), s)
         ka, kb = max(col0, _lib.T_KE), col1
         if ka < kb:
             call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
                  ptr(self.red_chunks), ptr(self.thermo), s)
+
+    def force_partials(self):
+        """(partial rows, per-replica row offsets) the force field writes."""
+        ff = self.force_field
+        if ff is not None and hasattr(ff, "partials"):
+            return ff.partials(self)
+        if self.use_tiled:
+            return self.partial_cell, self.rep_cells
+        return self.partial, self.rep_tiles
 
     def barostat(self):
         call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,

@@ -80,11 +80,16 @@ class EAMForceField:
             p.inv_drho[k] = float(np.float32(t.inv_drho[a]))
         self.params = p
 
-    def compute(self, eng, virial=True):
+    def compute(self, eng, virial=True, force=None, partial=None):
+        """force/partial: optional output arrays in place of the engine's
+        (float [N][3] and double [ntiles][16]; the global-list kernels run)."""
         rho = eng.scratch("eam_rho", torch.float32)
         fp = eng.scratch("eam_fp", torch.float32)
         s = stream_ptr()
-        if eng.use_tiled and self.params.ntypes == 1:
+        own = force is None and partial is None
+        force = eng.force if force is None else force
+        partial = eng.partial if partial is None else partial
+        if own and eng.use_tiled and self.params.ntypes == 1:
             common = (eng.ncells, ptr(eng.reps), ptr(eng.cell_rep), ptr(eng.cell_start),
                       ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16), ptr(eng.nnbr),
                       eng.max_nbr, eng.cap, self.params)
@@ -94,8 +99,7 @@ class EAMForceField:
                  int(bool(virial)), ptr(eng.overflow), s)
             return
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp),
-             ptr(eng.partial), s)
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp), ptr(partial), s)
         call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(fp), ptr(eng.force),
-             ptr(eng.partial), int(bool(virial)), s)
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(fp), ptr(force), ptr(partial),
+             int(bool(virial)), s)

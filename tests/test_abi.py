@@ -43,16 +43,23 @@ def test_argument_errors_without_gpu():
     assert lib.sim_reduce(0, None, None, 0, 1, None, None, None) == -1
     assert lib.sim_nl_trigger(1, None, None, None, None, 0, None) == -1
     assert lib.sim_pos_export(None, 10, None, 0, None) == -1
+    net = _lib.MLPNet()
+    assert lib.sim_mlp_energy(1, None, None, 1, None, None, None, 1, net, None, None, None, None,
+                              None) == -1
+    assert lib.sim_mlp_grad(0, None, None, None, None, 1.0, 1.0, net, None, None, None, None,
+                            None, 1, None, None) == -1
+    assert lib.sim_mlp_adam(None, None, None, None, 1, 1e-3, 0.9, 0.999, 1e-8, net, None) == -1
 
 
 def test_struct_layouts_match_c(tmp_path):
     from simulon_amd import _lib
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "simulon_hip.h"\n'
-                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(sim_replica_t),'
+                   'int main(){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %d %d\\n", sizeof(sim_replica_t),'
                    'sizeof(sim_types_t), sizeof(sim_lj_params_t), sizeof(sim_eam_params_t),'
                    'offsetof(sim_replica_t, atom_base), offsetof(sim_eam_params_t, inv_dr),'
-                   'offsetof(sim_replica_t, damp));return 0;}\n')
+                   'offsetof(sim_replica_t, damp), sizeof(sim_mlp_net_t),'
+                   'offsetof(sim_mlp_net_t, rc), SIM_MLP_NPARAM, SIM_MLP_NGRAD);return 0;}\n')
     exe = tmp_path / "sizes"
     r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                        capture_output=True, text=True)
@@ -61,5 +68,6 @@ def test_struct_layouts_match_c(tmp_path):
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     want = [ctypes.sizeof(_lib.Replica), ctypes.sizeof(_lib.Types), ctypes.sizeof(_lib.LJParams),
             ctypes.sizeof(_lib.EAMParams), _lib.Replica.atom_base.offset,
-            _lib.EAMParams.inv_dr.offset, _lib.Replica.damp.offset]
+            _lib.EAMParams.inv_dr.offset, _lib.Replica.damp.offset, ctypes.sizeof(_lib.MLPNet),
+            _lib.MLPNet.rc.offset, _lib.MLP_NPARAM, _lib.MLP_NGRAD]
     assert got == want

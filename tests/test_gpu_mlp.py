@@ -1,0 +1,191 @@
+"""GPU parity of the MLP-potential kernels against the float64 oracle, the
+weight gradient against oracle autograd, and training on EAM frames.
+
+Tolerances (stated here, measured against oracle/mlp.py in float64 with the
+same bfloat16 weights):
+  forces      max-atom |F - F64| / max |F64| <= 2e-5
+  energy      relative 1e-6;  virial tensor relative (to its largest entry) 1e-5
+  gradient    per parameter block |g - g64| / |g64| <= 2e-3 (float32 list
+              sums and hi/lo-split bfloat16 MFMA products)
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as om
+from simulon_amd import lattice as lat
+from simulon_amd.engine import ReplicaEngine, ReplicaSpec
+from simulon_amd.mlp import FrameSet, JudgedMLP, MLPForceField, MLPNet, Trainer
+from tests.helpers import force_error, sym3
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def _random_net(seed=0):
+    net = MLPNet("cuda", seed=seed)
+    rng = np.random.default_rng(seed)
+    flat = net.master.cpu().numpy().copy()
+    flat[2048:2112] = rng.normal(0, 0.3, 64)       # b1
+    flat[6208:6272] = rng.normal(0, 0.3, 64)       # b2
+    flat[6336] = -8.0
+    net.load_flat(flat)
+    net.set_scaling(rng.uniform(0.5, 4.0, 32), rng.uniform(0.5, 3.0, 32))
+    return net
+
+
+def _oracle_inputs(net):
+    mean = net.mean.double().cpu().numpy()
+    std = 1.0 / net.inv_std.double().cpu().numpy()
+    return om.to_torch(net.params_numpy()), mean, std
+
+
+def _system(tri, seed):
+    pos, H = lat.w_bcc(5, seed=seed, amp=0.15)
+    if tri:
+        H = H.copy()
+        H[1, 0] = 0.1 * H[0, 0]
+        H[2, 0] = 0.05 * H[0, 0]
+    return pos, H
+
+
+@pytest.mark.parametrize("tri", [False, True])
+def test_mlp_energy_forces_virial_vs_fp64(tri):
+    _print_case("mlp_forces", tri=tri)
+    pos, H = _system(tri, seed=4)
+    net = _random_net(1)
+    eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H, velocities=np.zeros_like(pos))],
+                        [lat.W_MASS], 5.5, 0.5, tiled=False)
+    eng.force_field = MLPForceField(net)
+    eng.setup()
+    x = eng.positions(torch.float64).cpu().numpy()
+    P, mean, std = _oracle_inputs(net)
+    E, F, _, _ = om.energy_forces(x, H, P, mean, std)
+    _, W, vir = om.energy_virial(x, H, P, mean, std)
+    th = eng.thermo_host()[0]
+    err = force_error(eng.force.cpu().numpy(), F.numpy())
+    print(f"force err {err:.3g}  E {th[0]:.9g} vs {float(E):.9g}", flush=True)
+    assert err <= 2e-5
+    assert abs(th[0] - float(E)) <= 1e-6 * abs(float(E))
+    Wd = sym3(th[2:8])
+    assert np.abs(Wd - W).max() <= 1e-5 * np.abs(W).max()
+    assert abs(th[1] - vir) <= 1e-5 * max(1.0, abs(vir))
+
+
+def test_mlp_batched_equals_single():
+    _print_case("mlp_batched")
+    net = _random_net(2)
+    systems = [_system(False, 10 + k) for k in range(3)]
+    specs = [ReplicaSpec(positions=p, box=h, velocities=np.zeros_like(p)) for p, h in systems]
+    big = ReplicaEngine(specs, [lat.W_MASS], 5.5, 0.5, tiled=False, sort=False)
+    big.force_field = MLPForceField(net)
+    big.setup()
+    for r, s in enumerate(specs):
+        one = ReplicaEngine([s], [lat.W_MASS], 5.5, 0.5, tiled=False, sort=False)
+        one.force_field = MLPForceField(net)
+        one.setup()
+        sl = big.replica_slice(r)
+        assert torch.equal(big.force[sl], one.force)
+        assert big.thermo_host()[r, 0] == one.thermo_host()[0, 0]
+
+
+def test_mlp_gradient_vs_oracle_autograd():
+    _print_case("mlp_grad")
+    net = _random_net(3)
+    frames = []
+    rng = np.random.default_rng(7)
+    for k in range(2):
+        pos, H = _system(k == 1, seed=20 + k)
+        frames.append((pos, H, -8.0 * len(pos) + rng.normal(0, 2.0),
+                       rng.normal(0, 0.5, pos.shape)))
+    fs = FrameSet([f[0] for f in frames], [f[1] for f in frames], [f[2] for f in frames],
+                  [f[3] for f in frames], lat.W_MASS, 5.5)
+    tr = Trainer(net, wE=0.3, wF=1.0)
+    tr.loss_and_grad(fs)
+    g = net.grad.double().cpu().numpy()
+    P, mean, std = _oracle_inputs(net)
+    L, grads = om.loss_and_grad(
+        [(p, h, e, f.astype(np.float32).astype(np.float64)) for p, h, e, f in frames],
+        [x.detach().numpy() for x in P], mean, std, wE=0.3, wF=1.0)
+    print(f"loss {g[-1]:.8g} vs {L:.8g}", flush=True)
+    assert abs(g[-1] - L) <= 1e-4 * abs(L)
+    offs = [(0, 2048), (2048, 2112), (2112, 6208), (6208, 6272), (6272, 6336), (6336, 6337)]
+    for (a, b), go in zip(offs, grads):
+        go = np.asarray(go, np.float64).ravel()
+        rel = np.linalg.norm(g[a:b] - go) / np.linalg.norm(go)
+        print(f"grad block {a}:{b} rel err {rel:.3g}", flush=True)
+        assert rel <= 2e-3
+
+
+def test_mlp_graph_matches_eager():
+    _print_case("mlp_graph")
+    pos, H = _system(False, seed=30)
+    net = _random_net(4)
+
+    def make():
+        e = ReplicaEngine([ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVE",
+                                       temperature=300.0)],
+                          [lat.W_MASS], 5.5, 0.8, tiled=False, velocity_seed=3)
+        e.force_field = MLPForceField(net)
+        e.setup()
+        return e
+
+    a, b = make(), make()
+    a.run(20, graph=False)
+    b.run(20, graph=True)
+    assert torch.equal(a.pos4, b.pos4)
+    assert torch.equal(a.force, b.force)
+
+
+@pytest.fixture(scope="module")
+def w_eam(tmp_path_factory):
+    from simulon_amd.forcefield import EAMForceField
+    p = tmp_path_factory.mktemp("eam") / "W_synthetic.eam.alloy"
+    return EAMForceField(str(lat.write_w_synthetic_setfl(p)))
+
+
+def test_training_gains_over_untrained_network_and_judge(w_eam):
+    """Train on EAM frames on the GPU; the trained network's force RMSE on
+    held-out frames must fall well below the untrained one's, and an MD run
+    judged by EAM every 10 steps must stay on the trained network while the
+    untrained one is handed over to EAM at its first judgement."""
+    from simulon_amd.mlp import generate_frames
+    _print_case("mlp_train")
+    pos, boxes, en, frc, Ts = generate_frames(w_eam, nframes=48, ncell=5, temps=(300.0, 2000.0),
+                                              nrep=8, equil=200, stride=10, seed=1)
+    train = [k for k in range(48) if k % 6 != 5]
+    held = [k for k in range(48) if k % 6 == 5]
+    sel = lambda ks: ([pos[k] for k in ks], [boxes[k] for k in ks], [en[k] for k in ks],
+                      [frc[k] for k in ks])
+    batches = []
+    for c in range(0, len(train), 10):
+        batches.append(FrameSet(*sel(train[c:c + 10]), lat.W_MASS, 5.5))
+    test = FrameSet(*sel(held), lat.W_MASS, 5.5)
+    net = MLPNet("cuda", seed=0)
+    tr = Trainer(net, wE=0.05, wF=1.0)
+    tr.fit_scaling(batches[0])
+    net.init_weights(0, energy_offset=float(np.mean([e / len(p) for e, p in zip(en, pos)])))
+    rmse0, ref = tr.force_rmse(test)
+    tr.fit(batches, epochs=60, lr=3e-3, lr_end=3e-4)
+    rmse1, _ = tr.force_rmse(test)
+    print(f"held-out force RMSE {rmse0:.4f} -> {rmse1:.4f} eV/A (force RMS {ref:.4f})", flush=True)
+    assert rmse1 < 0.25 * rmse0
+    assert rmse1 < 0.5 * ref
+
+    def judged(network):
+        p, h = lat.w_bcc(5, seed=5, amp=0.05)
+        eng = ReplicaEngine([ReplicaSpec(positions=p, box=h, dt=0.001, ensemble="NVT",
+                                         temperature=600.0)],
+                            [lat.W_MASS], 5.5, 1.0, tiled=False, velocity_seed=9)
+        j = JudgedMLP(eng, MLPForceField(network), w_eam, every=10, threshold=0.5 * ref)
+        eng.setup()
+        return j.run(40)
+
+    good = judged(net)
+    bad = judged(MLPNet("cuda", seed=1))
+    print(f"trained judge history {good.history}; untrained {bad.history}", flush=True)
+    assert good.on_mlp and good.fallback_step is None and len(good.history) == 4
+    assert not bad.on_mlp and bad.fallback_step == 10
