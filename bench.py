@@ -17,7 +17,9 @@ float64 oracle on sampled replicas, the roofline of the dominant kernel
 rate on a bounded sample (host cores).
 
 Other workloads (--workload): lj_argon_nvt (1000 Ar), w_eam_nve (54,000 W),
-lj_argon_npt_tri (1,048,576 Ar, triclinic, Berendsen + Langevin).
+lj_argon_npt_tri (1,048,576 Ar, triclinic, Berendsen + Langevin), mlp_w (the
+learned MLP potential trained on EAM frames, run on 128,000 W atoms with EAM
+as the judge every 50 steps, against plain EAM).
 """
 import argparse
 import json
@@ -301,6 +303,132 @@ def bench_single(args):
             "rebuilds": int(eng.nbuild.cpu()[0]), "roofline": roof}
 
 
+def mlp_kernel_roofline(eng):
+    """Roofline of the MLP passes with the algorithmic bytes per atom:
+    energy pass 12 (x) + 4 nbar (list) + 128 (dE/dg write); force pass
+    12 + 4 nbar + 128 (dE/dg read) + 12 (F write). Also the dense-layer MFMA
+    rate of the energy pass: 2 (32*64 + 64*64 + 64) flop forward and
+    2 (64*64 + 64*32) + 64 backward to the input per atom, twice for the
+    hi/lo activation split."""
+    from simulon_amd._lib import call, ptr, stream_ptr
+    ff = eng.force_field
+    nbar = mean_full_list(eng)
+    c = eng.scratch("mlp_c", torch.float32, (eng.N, 32))
+    part, _ = ff.partials(eng)
+    st = ff.net.struct()
+
+    def p1():
+        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), None, None, ptr(part), stream_ptr())
+
+    def p2():
+        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), ptr(eng.force), ptr(part), 0,
+             stream_ptr())
+
+    t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
+    b1 = eng.N * (12 + 4 * nbar + 128)
+    b2 = eng.N * (12 + 4 * nbar + 128 + 12)
+    name, t, b = ("mlp_force", t2, b2) if t2 >= t1 else ("mlp_energy", t1, b1)
+    ach = b / (t * 1e-3) / 1e9
+    flops = eng.N * (2 * (32 * 64 + 64 * 64 + 64) + 2 * (64 * 64 + 64 * 32) + 64)
+    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
+            "other_pass_ms": round(t1 if t == t2 else t2, 5),
+            "energy_pass_dense_tflops": round(flops / (t1 * 1e-3) / 1e12, 2)}
+
+
+def bench_mlp(args):
+    """Learned MLP potential vs plain EAM: frames from EAM Langevin runs of
+    2000-atom W cells at 300-3000 K, force-matching training on the GPU, then
+    a 128,000-atom W cell (40^3 bcc) at 600 K NVT on the MLP with EAM as the
+    judge every 50 steps, against the same run on plain EAM."""
+    from simulon_amd.mlp import FrameSet, JudgedMLP, MLPForceField, MLPNet, Trainer, \
+        generate_frames
+    tmp = tempfile.mkdtemp()
+    setfl = lat.write_w_synthetic_setfl(os.path.join(tmp, "W_synthetic.eam.alloy"))
+    eam = EAMForceField(setfl)
+    t0 = time.perf_counter()
+    pos, boxes, en, frc, Ts = generate_frames(eam, nframes=args.mlp_frames, ncell=10,
+                                              temps=(300.0, 3000.0), nrep=20, equil=0,
+                                              stride=20, seed=0)
+    t_gen = time.perf_counter() - t0
+    # every tenth sample time of every replica is held out
+    held = [k for k in range(len(pos)) if (k // 20) % 10 == 9]
+    train = [k for k in range(len(pos)) if (k // 20) % 10 != 9]
+
+    def sel(ks):
+        return ([pos[k] for k in ks], [boxes[k] for k in ks], [en[k] for k in ks],
+                [frc[k] for k in ks])
+
+    batches = [FrameSet(*sel(train[c:c + 100]), lat.W_MASS, 5.5)
+               for c in range(0, len(train), 100)]
+    test = FrameSet(*sel(held), lat.W_MASS, 5.5)
+    log(f"frames: {len(pos)} x {len(pos[0])} atoms in {t_gen:.1f} s; {len(batches)} batches")
+    net = MLPNet("cuda", seed=0)
+    tr = Trainer(net, wE=0.05, wF=1.0)
+    tr.fit_scaling(batches)
+    net.init_weights(0, energy_offset=float(np.mean([e / len(p) for e, p in zip(en, pos)])))
+    rmse0, _ = tr.force_rmse(test)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = tr.fit(batches, epochs=args.mlp_epochs, lr=3e-3, lr_end=1e-4)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    rmse1, ref = tr.force_rmse(test)
+    log(f"training: {args.mlp_epochs} epochs ({args.mlp_epochs * len(batches)} Adam steps) in "
+        f"{t_train:.2f} s; held-out force RMSE {rmse0:.4f} -> {rmse1:.4f} eV/A (RMS {ref:.4f})")
+    del batches, test
+    # 128k-atom inference, plain EAM vs MLP judged by EAM
+    p, H = lat.w_bcc(40, seed=0, amp=0.02)
+    spec = ReplicaSpec(positions=p, box=H, dt=0.001, ensemble="NVT", temperature=600.0,
+                       gamma=10.0)
+    e_eam = ReplicaEngine([spec], [lat.W_MASS], cutoff=5.5, skin=1.0, seed=7, velocity_seed=11)
+    e_eam.force_field = eam
+    e_eam.setup()
+    e_eam.run(args.warmup, graph=True, virial=False)
+    t_e = timed(lambda k: e_eam.run(k, graph=True, virial=False, check=False), args.steps)
+    e_eam.check()
+    rate_eam = e_eam.N * args.steps / t_e
+    e_mlp = ReplicaEngine([spec], [lat.W_MASS], cutoff=5.5, skin=1.0, seed=7, velocity_seed=11)
+    judge = JudgedMLP(e_mlp, MLPForceField(net), eam, every=50, threshold=args.mlp_threshold)
+    e_mlp.setup()
+    judge.run(50)
+    t_m = timed(lambda k: judge.run(k), args.steps)
+    rate_mlp = e_mlp.N * args.steps / t_m
+    t_pure = None
+    if judge.on_mlp:   # the MLP steps alone (graph replays, no judge)
+        t_pure = timed(lambda k: e_mlp.run(k, graph=True, virial=False, check=False), args.steps)
+    roof = mlp_kernel_roofline(e_mlp) if judge.on_mlp else None
+    rm = [h[1] for h in judge.history]
+    rr = [h[2] for h in judge.history]
+    log(f"EAM {t_e * 1e3 / args.steps:.3f} ms/step; MLP+judge {t_m * 1e3 / args.steps:.3f} ms/step; "
+        f"judge RMSE {rm}")
+    return {"metric": "atom-steps/s (mlp_w, learned MLP judged by EAM every 50 steps)",
+            "value": round(float(rate_mlp), 1), "unit": "atom-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(t_m * 1e3 / args.steps, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16-mfma/f32", "data": "synthetic",
+            "config": {"workload": "mlp_w", "atoms": e_mlp.N, "temperature_K": 600.0,
+                       "judge_every": 50, "judge_threshold_eV_A": args.mlp_threshold,
+                       "train_frames": len(train), "held_out_frames": len(held),
+                       "frame_atoms": len(pos[0]), "train_temperatures_K": [300, 3000]},
+            "plain_eam_atom_steps_per_s": round(float(rate_eam), 1),
+            "mlp_only_atom_steps_per_s": (round(float(e_mlp.N * args.steps / t_pure), 1)
+                                          if t_pure else None),
+            "mlp_vs_eam": round(float(rate_mlp / rate_eam), 3),
+            "ns_per_day": round(args.steps * 0.001e-3 * 86400 / t_m, 3),
+            "on_mlp_at_end": judge.on_mlp, "fallback_step": judge.fallback_step,
+            "judge_force_rmse_eV_A": round(float(np.mean(rm)), 5) if rm else None,
+            "judge_force_rms_eam_eV_A": round(float(np.mean(rr)), 5) if rr else None,
+            "held_out_force_rmse_eV_A": [round(rmse0, 5), round(rmse1, 5)],
+            "held_out_force_rms_eV_A": round(ref, 5),
+            "train_seconds": round(t_train, 3), "frame_gen_seconds": round(t_gen, 2),
+            "train_loss_first_last": [hist[0], hist[-1]],
+            "roofline": roof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +441,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--mlp-frames", type=int, default=2000)
+    ap.add_argument("--mlp-epochs", type=int, default=200)
+    ap.add_argument("--mlp-threshold", type=float, default=0.1)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -323,6 +454,8 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if args.workload == "w_eam_replicas":
         out = bench_replicas(args)
+    elif args.workload == "mlp_w":
+        out = bench_mlp(args)
     else:
         out = bench_single(args)
     print(json.dumps(out), flush=True)

@@ -139,7 +139,7 @@ def loss_and_grad(frames, params_np, mean, std, wE=1.0, wF=1.0):
     params = to_torch(params_np, requires_grad=True)
     L = loss(frames, params, mean, std, wE, wF)
     grads = torch.autograd.grad(L, params)
-    return float(L), [g.numpy() for g in grads]
+    return float(L.detach()), [g.numpy() for g in grads]
 
 
 def bf16_round(a):

@@ -461,15 +461,17 @@ class ReplicaEngine:
             self.check()
 
     def check(self):
-        """Raise if a neighbor list or an LDS stencil overflowed (host read; call
-        between chunks)."""
+        """Raise if a neighbor list overflowed or a cell grid became too small
+        for the cell-tiled kernels (host read; call between chunks). Stencils
+        larger than the LDS tile are not errors (those cells read global
+        memory); they are counted in profile["lds_overflow"]."""
         ov = int(self.overflow.item())
+        if ov & 2:
+            self.profile["lds_overflow"] = self.profile.get("lds_overflow", 0) + 1
+            self.overflow.fill_(ov & ~2)
         if ov & 1:
             raise RuntimeError(f"neighbor list overflow: an atom has more than {self.max_nbr} "
                                "neighbors; construct the engine with a larger max_nbr")
-        if ov & 2:
-            raise RuntimeError(f"cell stencil holds more than {self.cap} atoms (LDS capacity); "
-                               "construct the engine with tiled=False")
         if ov & 4:
             raise RuntimeError("a replica's cell grid fell below 3 cells per dimension; "
                                "construct the engine with tiled=False")

@@ -205,20 +205,29 @@ class Trainer:
         self.gpart = torch.zeros((self.max_blocks, NGRAD), dtype=torch.float32, device=net.device)
         self.ff = MLPForceField(net)
 
-    def fit_scaling(self, frames: FrameSet):
-        """Descriptor mean and std over the atoms of `frames` (one kernel pass
-        with identity scaling)."""
+    def fit_scaling(self, frames):
+        """Descriptor mean and std over every atom of the FrameSet(s) `frames`
+        (one kernel pass each with identity scaling). A std below 5% of the
+        largest is raised to it, so that descriptors that barely vary in the
+        data (centres no neighbor reaches) do not amplify rare values."""
+        sets = frames if isinstance(frames, (list, tuple)) else [frames]
         net = self.net
-        mean, inv = net.mean.clone(), net.inv_std.clone()
         net.mean.zero_()
         net.inv_std.fill_(1.0)
-        eng = frames.eng
-        g = eng.scratch("mlp_gh", torch.float32, (eng.N, NDESC))
-        self.ff.compute(eng, virial=False, gh=g)
-        m = g.double().mean(0)
-        sd = g.double().std(0).clamp_min(1e-3)
-        net.mean.copy_(mean)
-        net.inv_std.copy_(inv)
+        s1 = torch.zeros(NDESC, dtype=torch.float64, device=net.device)
+        s2 = torch.zeros_like(s1)
+        n = 0
+        for fs in sets:
+            eng = fs.eng
+            g = eng.scratch("mlp_gh", torch.float32, (eng.N, NDESC))
+            self.ff.compute(eng, virial=False, gh=g)
+            gd = g.double()
+            s1 += gd.sum(0)
+            s2 += (gd * gd).sum(0)
+            n += eng.N
+        m = s1 / n
+        sd = torch.sqrt(torch.clamp(s2 / n - m * m, min=0.0))
+        sd = torch.maximum(sd, 0.05 * sd.max()).clamp_min(1e-6)
         net.set_scaling(m.cpu().numpy(), sd.cpu().numpy())
 
     def loss_and_grad(self, frames: FrameSet):
@@ -277,11 +286,14 @@ class Trainer:
 
 
 def generate_frames(eam_ff, nframes=2000, ncell=10, temps=(300.0, 3000.0), nrep=20,
-                    equil=1000, stride=20, dt=0.001, gamma=10.0, seed=0, device="cuda"):
+                    equil=0, stride=20, dt=0.001, gamma=10.0, seed=0, device="cuda"):
     """Frames of bcc W cells (ncell^3 cells, 2 atoms each) sampled from
     Langevin runs of the EAM force field at temperatures evenly spaced over
     `temps`, all replicas advanced together: returns (positions, boxes,
-    energies, forces, temperatures) lists in the caller's atom order."""
+    energies, forces, temperatures) lists in the caller's atom order, frame
+    k = sample k // nrep of replica k % nrep. Sampling starts after `equil`
+    steps; the default 0 keeps the approach to equilibrium from the lattice
+    in the data (MD runs start there too)."""
     from .lattice import w_bcc, W_MASS
     per = int(np.ceil(nframes / nrep))
     Ts = np.linspace(temps[0], temps[1], nrep)

@@ -284,6 +284,40 @@ def test_tiled_matches_global_list_kernels(ff, w_setfl):
     np.testing.assert_allclose(th_t[2:8], th_g[2:8], rtol=0, atol=2e-6 * np.abs(th_g[2:8]).max())
 
 
+@pytest.mark.parametrize("ff", ["lj", "eam"])
+def test_tiled_stencil_overflow_reads_global_memory(ff, w_setfl):
+    """With an LDS tile too small for the stencils, every cell takes the
+    global-memory path through the same stencil table: same forces and sums,
+    bit for bit, and the overflow is reported but not raised."""
+    if ff == "lj":
+        pos, H = lat.lj_argon_npt_tri(ncell=8, seed=9)
+        mk = lambda: LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
+        mass, rc = lat.AR_MASS, 8.5
+    else:
+        pos, H = lat.w_bcc(10, seed=9, amp=0.1)
+        mk = lambda: EAMForceField(w_setfl)
+        mass, rc = lat.W_MASS, 5.5
+    out = []
+    for small in (False, True):
+        eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H)], [mass], cutoff=rc, skin=1.0,
+                            tiled=True)
+        eng.force_field = mk()
+        eng.build_list(force=True)
+        eng.pick_cap()
+        assert eng.use_tiled and int(eng.stencil_max.item()) > 512
+        if small:
+            eng.cap = 512
+        eng.compute_forces(virial=True)
+        eng.reduce(0, 8)
+        ov = int(eng.overflow.item())
+        assert bool(ov & 2) == small
+        eng.check()
+        assert eng.profile.get("lds_overflow", 0) == int(small)
+        out.append((eng.force.clone(), eng.thermo[0, :8].clone()))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+
+
 def test_rebuild_mid_step_keeps_atoms_consistent():
     """A rebuild between the two halves permutes every per-atom array (the
     half-step velocities included): an NVE run with frequent rebuilds on the

@@ -155,9 +155,9 @@ def test_training_gains_over_untrained_network_and_judge(w_eam):
     from simulon_amd.mlp import generate_frames
     _print_case("mlp_train")
     pos, boxes, en, frc, Ts = generate_frames(w_eam, nframes=48, ncell=5, temps=(300.0, 2000.0),
-                                              nrep=8, equil=200, stride=10, seed=1)
-    train = [k for k in range(48) if k % 6 != 5]
-    held = [k for k in range(48) if k % 6 == 5]
+                                              nrep=8, stride=10, seed=1)
+    train = [k for k in range(48) if (k // 8) % 6 != 5]
+    held = [k for k in range(48) if (k // 8) % 6 == 5]
     sel = lambda ks: ([pos[k] for k in ks], [boxes[k] for k in ks], [en[k] for k in ks],
                       [frc[k] for k in ks])
     batches = []
@@ -166,7 +166,7 @@ def test_training_gains_over_untrained_network_and_judge(w_eam):
     test = FrameSet(*sel(held), lat.W_MASS, 5.5)
     net = MLPNet("cuda", seed=0)
     tr = Trainer(net, wE=0.05, wF=1.0)
-    tr.fit_scaling(batches[0])
+    tr.fit_scaling(batches)
     net.init_weights(0, energy_offset=float(np.mean([e / len(p) for e, p in zip(en, pos)])))
     rmse0, ref = tr.force_rmse(test)
     tr.fit(batches, epochs=60, lr=3e-3, lr_end=3e-4)
