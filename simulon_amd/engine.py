@@ -132,7 +132,7 @@ class ReplicaEngine:
         self.tiled = can_tile if tiled is None else (bool(tiled) and can_tile)
         self.cap = 0
         # stencil populations only change at rebuilds; overflow is checked
-        self.cap_margin = 1.08
+        self.cap_margin = 1.05
         self.force_field = None
         self._scratch = {}
         self._graph = None
@@ -177,7 +177,9 @@ class ReplicaEngine:
 
     def pick_cap(self):
         """LDS capacity of the cell-tiled kernels from the largest 27-cell
-        stencil population of the last build (host read, setup only)."""
+        stencil population of the last build (host read, setup only); a
+        stencil that outgrows it later is read from global memory (slower,
+        still exact)."""
         if not self.tiled or self.cap:
             return
         smax = int(self.stencil_max.item())
@@ -280,7 +282,7 @@ class ReplicaEngine:
                 rho = self.natoms[r] / abs(np.linalg.det(np.asarray(s.box, np.float64)))
                 est = max(est, 4.0 / 3.0 * math.pi * self.rl ** 3 * rho)
             max_nbr = int(math.ceil((1.3 * est + 16) / 8.0) * 8)
-        self.max_nbr = int(max_nbr)
+        self.max_nbr = int(math.ceil(int(max_nbr) / 8.0) * 8)   # rows 16-byte aligned
         self.nbr = torch.zeros((self.max_nbr, N), **i32)
         self.nnbr = torch.zeros(N, **i32)
         self.overflow = torch.zeros(1, **i32)
@@ -367,14 +369,14 @@ class ReplicaEngine:
              ptr(noise), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
         call("sim_step_advance", ptr(self.step_ctr), s)
 
-    def reduce(self, col0=0, col1=15):
+    def reduce(self, col0=0, col1=15, ff=None):
         """float64 per-replica sums of the partial columns [col0, col1): force
-        columns (< T_KE) from the pair kernels' partials, kinetic columns from
-        the integrator's."""
+        columns (< T_KE) from the partials of force field ff (default: the
+        attached one), kinetic columns from the integrator's."""
         s = stream_ptr()
         fa, fb = col0, min(col1, _lib.T_KE)
         if fa < fb:
-            part, rows = self.force_partials()
+            part, rows = self.force_partials(ff)
             call("sim_reduce", self.R, ptr(rows), ptr(part), fa, fb, ptr(self.red_chunks),
                  ptr(self.thermo), s)
         ka, kb = max(col0, _lib.T_KE), col1
@@ -382,9 +384,9 @@ class ReplicaEngine:
             call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
                  ptr(self.red_chunks), ptr(self.thermo), s)
 
-    def force_partials(self):
+    def force_partials(self, ff=None):
         """(partial rows, per-replica row offsets) the force field writes."""
-        ff = self.force_field
+        ff = self.force_field if ff is None else ff
         if ff is not None and hasattr(ff, "partials"):
             return ff.partials(self)
         if self.use_tiled:

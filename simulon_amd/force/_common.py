@@ -8,7 +8,7 @@ from .. import _lib
 def backbone_outputs(mol, ff, virial=True):
     eng = mol.engine
     ff.compute(eng, virial)
-    eng.reduce(0, 8)
+    eng.reduce(0, 8, ff=ff)
     th = eng.thermo[0]
     return {
         "energy": th[_lib.T_EPOT],

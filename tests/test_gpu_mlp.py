@@ -189,3 +189,21 @@ def test_training_gains_over_untrained_network_and_judge(w_eam):
     print(f"trained judge history {good.history}; untrained {bad.history}", flush=True)
     assert good.on_mlp and good.fallback_step is None and len(good.history) == 4
     assert not bad.on_mlp and bad.fallback_step == 10
+
+
+def test_mlp_backbone_return_contract():
+    """MLPForce on an AtomFileReader returns the learned-potential dict of
+    machine_learning_force.py:85-142 (+ virial_tensor) in the caller's order."""
+    from simulon_amd.force import MLPForce
+    from simulon_amd.reader import AtomFileReader
+    _print_case("mlp_backbone")
+    pos, H = _system(False, seed=40)
+    mol = AtomFileReader(box_length=float(H[0, 0]), cutoff=5.5, device="cuda",
+                         atom_types=["W"] * len(pos), positions=pos, skin_thickness=1.0)
+    net = _random_net(5)
+    out = MLPForce(mol, net).forward()
+    assert {"energy", "forces", "virial", "virial_tensor"} <= set(out)
+    P, mean, std = _oracle_inputs(net)
+    E, F, _, _ = om.energy_forces(pos, H, P, mean, std)
+    assert force_error(out["forces"].cpu().numpy(), F.numpy()) <= 2e-5
+    assert abs(float(out["energy"]) - float(E)) <= 1e-6 * abs(float(E))
