@@ -14,8 +14,8 @@ PyTorch CPU path for one MD step:
                  core/force/eam_force_cu.py:334-579  tables + two-pass EAM
   integrate.py   core/integrator/integrator.py:45-124,
                  io_utils/reader.py:319-339,
-                 core/md_model.py:83-166             BAOB Langevin velocity Verlet
-  barostat.py    core/barostat.py:36-68              Berendsen
+                 core/md_model.py:83-166,            BAOB Langevin velocity Verlet,
+                 core/barostat.py:36-68              Berendsen scaling factor
   mlp.py         radial-descriptor MLP potential (no reference counterpart:
                  the reference's learned potential is an external CHGNet
                  wrapper; this restates the library's own network in FP64)

@@ -15,7 +15,7 @@ numpy performs every elementwise op with one IEEE rounding and no
 contraction, and np.rint rounds half to even like torch.round.
 
 pinned by: closed-form cases and invariants (|s|<=1/2, idempotence,
-agreement with a 27-image brute force) in tests/test_oracle_box.py, and the
+agreement with a 27-image brute force) in tests/test_oracle_geometry.py, and the
 reference's own FP32/FP64 minimum-image outputs in tests/golden/.
 """
 import numpy as np

@@ -18,9 +18,9 @@ Tables are evaluated in the requested dtype: float32 reproduces the
 reference's float32 knots/coefficients (`_spline_eval`, eam_force_cu.py:397-415),
 float64 evaluates the same splines in double precision for the FP64 run.
 
-pinned by: analytic tables (synthetic file) within the linear-interpolation
-bound, F = -dE/dx by FP64 central differences (table-consistent derivative
-tables), hand-computed 2- and 3-atom cases, the reference's zero-stress and
+pinned by (tests/test_oracle_physics.py): analytic tables (synthetic file)
+within the linear-interpolation bound, a hand-computed two-atom case and a
+closed-form analytic EAM evaluated independently, the reference's zero-stress and
 elastic-constant regressions on run_data/W/WRe_YC2.eam.fs
 (cuda_test/test_eam_fs_parser_regression.py, cuda_test/test_w_elastic_static.py),
 and the reference's own tables and outputs (tests/golden/).

@@ -17,7 +17,7 @@ generator are tested separately.
 
 pinned by: the ballistic closed form under constant force, harmonic-oscillator
 energy conservation, the injected-noise formula, equipartition and the
-Berendsen closed form (tests/test_oracle_integrate.py).
+Berendsen closed form (tests/test_oracle_physics.py).
 """
 import numpy as np
 

@@ -15,7 +15,7 @@ Per-edge epsilon and sigma come from a per-type-pair table indexed by the
 two atom types (reader.py:165-210 builds them from the parameter dict).
 
 pinned by: the two-atom closed form, Newton's third law, F = -dE/dx by FP64
-central differences, tr(W) = virial, lattice sums (tests/test_oracle_lj.py)
+central differences, tr(W) = virial, lattice sums (tests/test_oracle_physics.py)
 and the reference's own outputs (tests/golden/).
 """
 import numpy as np

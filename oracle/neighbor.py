@@ -13,7 +13,7 @@ comparison is then made in the dtype of the squared distances.
 Squared distance order: (x*x + y*y) + z*z, one rounding per op.
 
 pinned by: brute force over 27 images, shell counts of perfect simple-cubic,
-bcc and fcc lattices (tests/test_oracle_neighbor.py) and the reference's own
+bcc and fcc lattices (tests/test_oracle_geometry.py) and the reference's own
 pair sets for seeded FP32 and FP64 inputs (tests/golden/).
 """
 import numpy as np
