@@ -292,6 +292,16 @@ def bench_single(args):
     t = timed(lambda k: eng.run(k, graph=True, virial=virial, check=False), args.steps)
     eng.check()
     rate = eng.N * args.steps / t
+    drift = None
+    if spec.ensemble == "NVE":
+        # total-energy drift over 2000 further steps (2 ps at 1 fs)
+        th = eng.thermo_host()[0]
+        e0 = th[0] + th[8]
+        eng.run(2000, graph=True, virial=False)
+        th = eng.thermo_host()[0]
+        drift = ((th[0] + th[8]) - e0) / eng.N / (2000 * dt_ps * 1e-3)
+        log(f"NVE: E/atom {e0 / eng.N:.6f} -> {(th[0] + th[8]) / eng.N:.6f} eV, drift "
+            f"{drift:.3e} eV/atom/ns")
     roof = lj_kernel_roofline(eng) if isinstance(eng.force_field, LJForceField) else \
         eam_kernel_roofline(eng)
     return {"metric": f"atom-steps/s ({w})", "value": round(float(rate), 1), "unit": "atom-steps/s",
@@ -300,7 +310,8 @@ def bench_single(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": w, "atoms": eng.N},
             "ns_per_day": round(args.steps * dt_ps * 1e-3 * 86400 / t, 3),
-            "rebuilds": int(eng.nbuild.cpu()[0]), "roofline": roof}
+            "rebuilds": int(eng.nbuild.cpu()[0]), "roofline": roof,
+            "nve_drift_eV_per_atom_per_ns": drift}
 
 
 def mlp_kernel_roofline(eng):
