@@ -77,25 +77,24 @@ def eam_kernel_roofline(eng):
     """Roofline of the EAM passes with the algorithmic bytes per atom:
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
     read) + 12 (F write). The dominant (longer) pass is reported; the kernels
-    timed are the ones the step launches (cell-tiled or global-list)."""
+    timed are the ones the step launches (row kernels or global-list)."""
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
 
     rho = eng.scratch("eam_rho", torch.float32)
     fp = eng.scratch("eam_fp", torch.float32)
-    if eng.use_tiled:
-        common = (eng.ncells, ptr(eng.reps), ptr(eng.cell_rep), ptr(eng.cell_start),
-                  ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16), ptr(eng.nnbr),
-                  eng.max_nbr, eng.cap, ff.params)
+    fam = eng.family if eng.use_tiled else "global"
+    if fam != "global":
+        common = (*(eng.row_args() if fam == "rows" else eng.cell_args()), ff.params)
 
         def p1():
-            call("sim_eam_density_cell", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
+            call(f"sim_eam_density_{fam}", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
                  ptr(eng.overflow), stream_ptr())
 
         def p2():
-            call("sim_eam_force_cell", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell), 1,
-                 ptr(eng.overflow), stream_ptr())
+            call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
+                 1, ptr(eng.overflow), stream_ptr())
     else:
         def p1():
             call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
@@ -110,8 +109,8 @@ def eam_kernel_roofline(eng):
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
     b1 = eng.N * (12 + 4 * nbar + 4)
     b2 = eng.N * (12 + 4 * nbar + 4 + 12)
-    pre = "eam_force_cell" if eng.use_tiled else "eam_force"
-    pre1 = "eam_density_cell" if eng.use_tiled else "eam_density"
+    pre = "eam_force" + ("" if fam == "global" else "_" + fam)
+    pre1 = "eam_density" + ("" if fam == "global" else "_" + fam)
     name, t, b = (pre, t2, b2) if t2 >= t1 else (pre1, t1, b1)
     ach = b / (t * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
@@ -131,7 +130,7 @@ def lj_kernel_roofline(eng):
     t = kernel_time_ms(k)
     b = eng.N * (12 + 4 * nbar + 12)
     ach = b / (t * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": "lj_force_cell" if eng.use_tiled else "lj_force", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+    return {"bound": "hbm", "kernel": "lj_force" + (f"_{eng.family}" if eng.use_tiled else ""), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2)}
 
@@ -214,7 +213,7 @@ def bench_replicas(args):
     eng.run(args.warmup, graph=True, virial=False)
     t = timed(lambda k: eng.run(k, graph=True, virial=False, check=False), args.steps)
     eng.check()
-    log(f"pair kernels: {'cell-tiled (LDS cap %d)' % eng.cap if eng.use_tiled else 'global list'}")
+    log(f"pair kernels: {eng.family} (LDS cap {eng.cap}); autotune {eng.profile.get('autotune_ms')}")
     batched = natoms.sum() * args.steps / t
     builds = eng.nbuild.cpu().numpy()
     log(f"batched: {t * 1e3 / args.steps:.3f} ms/step, {batched:.4e} atom-steps/s, "

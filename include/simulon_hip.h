@@ -145,8 +145,8 @@ int sim_version(char* buf, int len);
  *   ro: optional physical reorder (NULL: atoms keep their slots). With ro, the
  *       atoms of each rebuilding replica are permuted into cell order and
  *       wrapped into the primary cell before the list is built, and the list is
- *       also written as 16-bit slots of the 27-cell stencil (cell-tiled kernels;
- *       requires >= 3 cells per dimension). */
+ *       also written as 16-bit column slots (row kernels; requires >= 3 cells
+ *       per dimension) and the cell-row prefix row_base is updated. */
 typedef struct sim_nl_reorder {
     float* vel;            /* [N][3], permuted with the atoms */
     float* force;          /* [N][3], permuted with the atoms */
@@ -159,10 +159,12 @@ typedef struct sim_nl_reorder {
     long long* order_tmp;
     int* image_tmp;
     int* cell_tmp;
-    unsigned short* nbr16; /* [N][max_nbr] row-major list of LDS slots, or NULL */
-    int* stencil_max;      /* largest 27-cell stencil population (atomic max), or NULL */
+    unsigned short* nbr16; /* [N][max_nbr] row-major list of column slots, or NULL */
+    int* stencil_max;      /* [2] largest stencil-column and 27-cell stencil populations
+                              (atomic max), or NULL */
     float* vel_half;       /* [N][3] half-step velocities (a rebuild falls between the halves) */
     float* vel_half_tmp;
+    int* row_base;         /* [R+1] prefix of the cell-row counts ny*nz, or NULL */
 } sim_nl_reorder_t;
 
 int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
@@ -220,12 +222,13 @@ int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
                   sim_eam_params_t eam, const float* fp, float* force, double* partial,
                   int virial, void* stream);
 
-/* Cell-tiled variants (one workgroup per cell slot, ncells = total cell
- * capacity): the 27-cell neighbourhood of the cell is staged in LDS (at most
- * cap atoms, cap <= 2048; overflow bit 2 is set when a stencil holds more) and
- * the lists are read as 16-bit slots (nbr16 from sim_nl_build with a reorder).
- * Partials are written per cell slot: partial[ncells][16]. The EAM variants
- * take one element type. */
+/* Cell variants (one workgroup per cell slot, ncells = total cell
+ * capacity): the 27-cell neighbourhood of the cell is staged in LDS column by
+ * column (at most cap atoms, cap <= 2048; a larger one is read from global
+ * memory instead and sets overflow bit 2) and the lists are read as the
+ * 16-bit column slots of nbr16. Partials are written per cell slot:
+ * partial[ncells][16]. The EAM variants take one element type. Suited to
+ * small systems; the row variants below to large ones. */
 int sim_lj_force_cell(int ncells, const void* reps, const int* cell_rep, const int* cell_start,
                       const int* cell_count, const double* pos4, const unsigned short* nbr16,
                       const int* nnbr, int max_nbr, int cap, sim_lj_params_t lj, float* force,
@@ -240,6 +243,30 @@ int sim_eam_force_cell(int ncells, const void* reps, const int* cell_rep, const 
                        const int* nnbr, int max_nbr, int cap, sim_eam_params_t eam,
                        const float* fp, float* force, double* partial, int virial, int* overflow,
                        void* stream);
+
+/* Row variants: one workgroup per row of cells along x (ncells = total
+ * cell capacity; row_base[R+1] from sim_nl_build with a reorder). A ring of
+ * four stencil columns (9 cells each) is staged in LDS, at most cap atoms per
+ * column (cap <= 512; a larger column is read from global memory instead and
+ * sets overflow bit 2), and the lists are read as the 16-bit column slots of
+ * nbr16. Partials are written per cell slot: partial[ncells][16] (zeros for
+ * slots outside a replica's current grid). The EAM variants take one element
+ * type; ntypes selects the LJ type handling. */
+int sim_lj_force_rows(int ncells, const void* reps, const int* cell_rep, const int* row_base,
+                      int R, const int* cell_start, const int* cell_count, const double* pos4,
+                      const unsigned short* nbr16, const int* nnbr, int max_nbr, int cap,
+                      int ntypes, sim_lj_params_t lj, float* force, double* partial, int virial,
+                      int* overflow, void* stream);
+int sim_eam_density_rows(int ncells, const void* reps, const int* cell_rep, const int* row_base,
+                         int R, const int* cell_start, const int* cell_count, const double* pos4,
+                         const unsigned short* nbr16, const int* nnbr, int max_nbr, int cap,
+                         sim_eam_params_t eam, float* rho, float* fp, double* partial,
+                         int* overflow, void* stream);
+int sim_eam_force_rows(int ncells, const void* reps, const int* cell_rep, const int* row_base,
+                       int R, const int* cell_start, const int* cell_count, const double* pos4,
+                       const unsigned short* nbr16, const int* nnbr, int max_nbr, int cap,
+                       sim_eam_params_t eam, const float* fp, float* force, double* partial,
+                       int virial, int* overflow, void* stream);
 
 /* ---- integrator --------------------------------------------------------- */
 

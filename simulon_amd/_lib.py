@@ -71,7 +71,8 @@ class NLReorder(ctypes.Structure):
     """Mirror of sim_nl_reorder_t."""
     _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
                                    "vel_tmp", "force_tmp", "order_tmp", "image_tmp", "cell_tmp",
-                                   "nbr16", "stencil_max", "vel_half", "vel_half_tmp")]
+                                   "nbr16", "stencil_max", "vel_half", "vel_half_tmp",
+                                   "row_base")]
 
 
 # name -> (restype, argtypes)
@@ -86,6 +87,12 @@ SIGNATURES = {
                                   _vp, _vp, _vp, _vp]),
     "sim_eam_force_cell": (_I, [_I, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _I, _I, EAMParams, _vp, _vp,
                                 _vp, _I, _vp, _vp]),
+    "sim_lj_force_rows": (_I, [_I, _vp, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp, _I, _I, _I,
+                               LJParams, _vp, _vp, _I, _vp, _vp]),
+    "sim_eam_density_rows": (_I, [_I, _vp, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp, _I, _I,
+                                  EAMParams, _vp, _vp, _vp, _vp, _vp]),
+    "sim_eam_force_rows": (_I, [_I, _vp, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp, _I, _I,
+                                EAMParams, _vp, _vp, _vp, _I, _vp, _vp]),
     "sim_nl_build_xyz": (_I, [_vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _vp, _I, _vp, _vp, _I, _vp,
                               _vp]),
     "sim_nl_half_count": (_I, [_vp, _vp, _I, _I, _vp, _vp, _vp]),

@@ -14,11 +14,13 @@
 //            coordinates (triclinic-safe), capped by the replica's capacity
 //   bin      cell id of each atom from its wrapped fractional coordinate,
 //            slot within the cell by an integer atomic counter
-//   scan     exclusive prefix over all cells (one workgroup)
+//   scan     exclusive prefix of the cell counts (one workgroup per replica)
 //   scatter  atom -> cell_atoms[start + slot]
 //   order    per-cell insertion sort by atom index (deterministic lists)
 //   list     thread per atom over the 27 (or fewer, when a dimension has < 3
-//            cells) distinct stencil cells; column-major full list
+//            cells) distinct stencil cells; column-major full list, and with
+//            a reorder also the 16-bit column slots of the row kernels
+//   rows     per-replica prefix of the cell-row counts (ny * nz)
 #include "sim_common.h"
 
 namespace {
@@ -102,22 +104,11 @@ __global__ void k_bin(const sim_replica_t* reps, const SimTile* tiles, const T* 
     atom_slot[i] = atomicAdd(&cell_count[cell], 1);
 }
 
-// Exclusive scan of cell_count[0..n) into cell_start[0..n], one workgroup of
-// 1024 threads, each owning a contiguous chunk.
-__global__ void __launch_bounds__(1024) k_scan(const int* cnt, int n, int* start, const int* flag,
-                                               int R) {
+// Exclusive scan of cnt[0..n) into start[0..n], one workgroup of 1024
+// threads, each owning a contiguous chunk (half-list counts).
+__global__ void __launch_bounds__(1024) k_scan_flat(const int* cnt, int n, int* start) {
     __shared__ int part[1024];
-    __shared__ int any;
     const int tid = threadIdx.x;
-    if (flag != nullptr) {
-        // nothing to do when no replica rebuilds: counts and starts are unchanged
-        if (tid == 0) any = 0;
-        __syncthreads();
-        for (int r = tid; r < R; r += 1024)
-            if (flag[r]) any = 1;
-        __syncthreads();
-        if (!any) return;
-    }
     const int chunk = (n + 1023) / 1024;
     const int b = tid * chunk, e = min(n, b + chunk);
     int s = 0;
@@ -136,6 +127,67 @@ __global__ void __launch_bounds__(1024) k_scan(const int* cnt, int n, int* start
         run += cnt[k];
     }
     if (tid == 1023) start[n] = part[1023];
+}
+
+// Exclusive prefix of the cell counts, one workgroup per rebuilding replica
+// over its own cell range: replica r's atoms occupy [atom_base, atom_base +
+// natoms), so its cell starts begin at atom_base and the other replicas'
+// starts stay valid.
+__global__ void __launch_bounds__(1024) k_scan(const sim_replica_t* reps, const int* cnt,
+                                               int* start, const int* flag) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int r = blockIdx.x;
+    if (!rep_active(flag, r)) return;
+    const sim_replica_t& P = reps[r];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int cb = P.cell_base, n = P.cell_cap;
+    if (tid == 0) carry = P.atom_base;
+    __syncthreads();
+    for (int b0 = 0; b0 < n; b0 += 1024) {
+        const int k = b0 + tid;
+        const int v = k < n ? cnt[cb + k] : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        if (wid == 0) {
+            int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const int t = __shfl_up(w, o, 64);
+                if (lane >= o) w += t;
+            }
+            if (lane < 16) wsum[lane] = w;   // inclusive over waves
+        }
+        __syncthreads();
+        const int base = carry + (wid > 0 ? wsum[wid - 1] : 0);
+        if (k < n) start[cb + k] = base + incl - v;
+        __syncthreads();
+        if (tid == 0) carry += wsum[15];
+        __syncthreads();
+    }
+    if (tid == 0) start[cb + n] = carry;   // = next replica's first start (or N)
+}
+
+// row_base[r] = number of cell rows (ny * nz) of replicas before r; rows of
+// cells along x are the work items of the row pair kernels.
+__global__ void k_row_base(int R, const sim_replica_t* reps, const int* flag, int* row_base) {
+    if (flag != nullptr) {
+        bool any = false;
+        for (int r = 0; r < R; ++r) any = any || flag[r] != 0;
+        if (!any) return;
+    }
+    int acc = 0;
+    for (int r = 0; r < R; ++r) {
+        row_base[r] = acc;
+        acc += reps[r].ncell[1] * reps[r].ncell[2];
+    }
+    row_base[R] = acc;
 }
 
 __global__ void k_scatter(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
@@ -269,7 +321,10 @@ __global__ void __launch_bounds__(SIM_TILE)
     int ox[3], oy[3], oz[3];
     const int sx = stencil(nx, ox), sy = stencil(ny, oy), sz = stencil(nz, oz);
     int cnt = 0;
-    int soff = 0;   // running LDS offset of the current stencil cell (cell-tiled kernels)
+    // Slots for the row kernels: (x column 0..2 of the stencil) << 14 | offset
+    // of the atom within that column's 9 cells, the cells taken in (z, y)
+    // order (the visiting order below), so a column is one contiguous LDS run.
+    int coff[3] = {0, 0, 0};
     for (int a = 0; a < sz; ++a) {
         int zz = cz + oz[a];
         zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
@@ -294,18 +349,23 @@ __global__ void __launch_bounds__(SIM_TILE)
                             nbr[(size_t)cnt * n_stride + i] = j;
                             if (nbr16 != nullptr)
                                 nbr16[(size_t)i * max_nbr + cnt] =
-                                    (unsigned short)(soff + (s - s0));
+                                    (unsigned short)((c << 14) | (coff[c] + (s - s0)));
                         }
                         ++cnt;
                     }
                 }
-                soff += s1 - s0;
+                coff[c] += s1 - s0;
             }
         }
     }
     nnbr[i] = cnt < max_nbr ? cnt : max_nbr;
     if (cnt > max_nbr) atomicOr(overflow, 1);
-    if (stencil_max != nullptr) atomicMax(stencil_max, soff);
+    if (stencil_max != nullptr) {
+        const int cmax = max(coff[0], max(coff[1], coff[2]));
+        atomicMax(stencil_max, cmax);                               // largest column
+        atomicMax(stencil_max + 1, coff[0] + coff[1] + coff[2]);    // largest stencil
+        if (cmax > 16383) atomicOr(overflow, 8);   // offsets no longer fit 14 bits
+    }
     if (pos_last != nullptr) {
         pos_last[(size_t)i * 4 + 0] = (double)x[(size_t)i * STRIDE + 0];
         pos_last[(size_t)i * 4 + 1] = (double)x[(size_t)i * STRIDE + 1];
@@ -325,7 +385,9 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                        ncells, flag, cell_count);
     hipLaunchKernelGGL((k_bin<T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, x,
                        flag, cell_count, atom_cell, atom_slot);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, ncells, cell_start, flag, R);
+    hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag);
+    if (ro != nullptr && ro->row_base != nullptr)
+        hipLaunchKernelGGL(k_row_base, dim3(1), dim3(1), 0, st, R, reps, flag, ro->row_base);
     hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                        atom_cell, atom_slot, cell_start, cell_atoms);
     hipLaunchKernelGGL(k_cell_order, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
@@ -452,7 +514,7 @@ extern "C" int sim_nl_half_count(const int* nbr, const int* nnbr, int n, int max
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_half_count, dim3((n + 255) / 256), dim3(256), 0, st, nbr, nnbr, n,
                        half_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, half_count, n, half_start, nullptr, 0);
+    hipLaunchKernelGGL(k_scan_flat, dim3(1), dim3(1024), 0, st, half_count, n, half_start);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

@@ -3,8 +3,12 @@
 //
 // After a rebuild the atoms of every cell are contiguous (sim_nl_build with a
 // reorder), positions are wrapped into the primary cell, and each atom's
-// Verlet list also exists as 16-bit slots into the concatenation of its cell's
-// 27 stencil cells (order z, y, x offsets -1..1). One workgroup takes one cell:
+// Verlet list also exists as 16-bit column slots (stencil column x-1, x, x+1
+// and the offset within that column's 9 cells). The halo concatenates the 27
+// stencil cells column by column, so a slot maps to a halo index with one
+// add. These per-cell kernels suit small systems (many short workgroups);
+// pair_rows.hip walks rows of cells for large ones. One workgroup takes one
+// cell:
 //   1. the 27 stencil cells' ranges, periodic image translations and LDS
 //      offsets go to LDS;
 //   2. all stencil atoms are copied into LDS with coalesced loads, as float64
@@ -40,7 +44,8 @@ struct Halo {
 };
 
 // Stencil metadata and halo positions for `cell`. The stencil cells are the
-// 27 neighbours in fixed (z, y, x) order; their periodic translation is
+// 27 neighbours in fixed (x, z, y) order, i.e. column by column, matching the
+// column slots of the lists; their periodic translation is
 // floor((c + o) / n) lattice vectors. Returns HALO_SKIP when the grid has
 // fewer than 3 cells in a dimension (overflow bit 4), HALO_LDS when the
 // stencil atoms are in LDS, HALO_GLOBAL when the stencil holds more than CAP
@@ -63,7 +68,7 @@ __device__ __forceinline__ int load_halo(Halo<CAP>& H, const sim_replica_t& P, i
     }
     if (tid < 27) {
         const int cx = local % nx, cy = (local / nx) % ny, cz = local / (nx * ny);
-        const int oz = tid / 9 - 1, oy = (tid / 3) % 3 - 1, ox = tid % 3 - 1;
+        const int ox = tid / 9 - 1, oz = (tid / 3) % 3 - 1, oy = tid % 3 - 1;
         int xx = cx + ox, yy = cy + oy, zz = cz + oz;
         const int fx = xx < 0 ? -1 : (xx >= nx ? 1 : 0);
         const int fy = yy < 0 ? -1 : (yy >= ny ? 1 : 0);
@@ -138,6 +143,14 @@ __device__ __forceinline__ void slot_get(const Halo<CAP>& H, const double4* __re
     a = AUX ? aux[j] : (float)p.w;
 }
 
+// Column slot (stencil column << 14 | offset within the column's 9 cells,
+// sim_nl_build) -> flat index into the halo, whose cells are stored column by
+// column in the same (z, y) order.
+template <int CAP>
+__device__ __forceinline__ int flat_slot(const Halo<CAP>& H, int s16) {
+    return H.off[(s16 >> 14) * 9] + (s16 & 0x3fff);
+}
+
 template <int LANES>
 __device__ __forceinline__ float lane_sum(float v) {
 #pragma unroll
@@ -207,7 +220,7 @@ __device__ __forceinline__ double eam_density_home(const Halo<CAP>& H,
                 float4 v[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int s = row[k + u * LANES];
+                    const int s = flat_slot(H, row[k + u * LANES]);
                     float rx, ry, rz, aj;
                     disp_slot<FROM_LDS, 0>(H, pos, nullptr, xi, yi, zi, s, rx, ry, rz, aj);
                     r2[u] = sq3(rx, ry, rz);
@@ -223,7 +236,7 @@ __device__ __forceinline__ double eam_density_home(const Halo<CAP>& H,
                 }
             }
             for (; k < cnt; k += LANES) {
-                const int s = row[k];
+                const int s = flat_slot(H, row[k]);
                 float rx, ry, rz, aj;
                 disp_slot<FROM_LDS, 0>(H, pos, nullptr, xi, yi, zi, s, rx, ry, rz, aj);
                 const float r2 = sq3(rx, ry, rz);
@@ -336,7 +349,7 @@ __device__ __forceinline__ void eam_force_home(double (&acc)[7], const Halo<CAP>
                 float4 va[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int s = row[k + u * LANES];
+                    const int s = flat_slot(H, row[k + u * LANES]);
                     disp_slot<FROM_LDS, 1>(H, pos, fp, xi, yi, zi, s, rx[u], ry[u], rz[u], fpj[u]);
                     r2[u] = sq3(rx[u], ry[u], rz[u]);
                     L[u] = lerp_idx(__builtin_sqrtf(r2[u]), E);
@@ -346,7 +359,7 @@ __device__ __forceinline__ void eam_force_home(double (&acc)[7], const Halo<CAP>
                 for (int u = 0; u < U; ++u) pair(rx[u], ry[u], rz[u], r2[u], L[u], va[u], fpj[u]);
             }
             for (; k < cnt; k += LANES) {
-                const int s = row[k];
+                const int s = flat_slot(H, row[k]);
                 float rx, ry, rz, fpj;
                 disp_slot<FROM_LDS, 1>(H, pos, fp, xi, yi, zi, s, rx, ry, rz, fpj);
                 const float r2 = sq3(rx, ry, rz);
@@ -425,7 +438,7 @@ __device__ __forceinline__ void lj_home(double (&acc)[8], const Halo<CAP>& H,
             const int cnt = nnbr[i];
             const unsigned short* row = nbr16 + (size_t)i * max_nbr;
             for (int k = l; k < cnt; k += LANES) {
-                const int s = row[k];
+                const int s = flat_slot(H, row[k]);
                 float rx, ry, rz, aj;
                 disp_slot<FROM_LDS, 0>(H, pos, nullptr, xi, yi, zi, s, rx, ry, rz, aj);
                 const float r2 = sq3(rx, ry, rz);

@@ -104,11 +104,22 @@ class ReplicaSpec:
     seed: int = 0
 
 
+FAMILIES = ("cell", "rows", "global")
+ROW_CAPS = (192, 256, 320, 384, 512)          # atoms per stencil column (row kernels)
+CELL_CAPS = (512, 1024, 1280, 1536, 2048)     # atoms per 27-cell stencil (cell kernels)
+
+
 class ReplicaEngine:
     """Packed multi-replica MD state and step on one GPU."""
 
     def __init__(self, specs, masses_amu, cutoff, skin, device="cuda", max_nbr=None,
-                 precise_list=True, seed=12345, velocity_seed=None, sort=True, tiled=None):
+                 precise_list=True, seed=12345, velocity_seed=None, sort=True, tiled=None,
+                 family=None):
+        """tiled / family choose the pair-kernel family: family "cell" (one
+        workgroup per cell, 27-cell halo in LDS), "rows" (one workgroup per row
+        of cells, sliding LDS window of stencil columns) or "global" (one
+        thread per atom over the int32 list); tiled=True means "rows",
+        tiled=False "global"; both None = measured choice at setup()."""
         self.device = torch.device(device)
         if isinstance(specs, ReplicaSpec):
             specs = [specs]
@@ -125,12 +136,18 @@ class ReplicaEngine:
         _lib.load()
         self._layout()
         self._alloc(max_nbr, velocity_seed)
-        # cell-tiled pair kernels need >= 3 cells per dimension in every replica
-        can_tile = all(min(self._grid_of(r)) >= 3 for r in range(self.R))
-        # None = automatic: tiled when the stencils fit the 1024-atom LDS tile
-        self._tiled_auto = tiled is None
-        self.tiled = can_tile if tiled is None else (bool(tiled) and can_tile)
-        self.cap = 0
+        # the LDS-staged kernels need >= 3 cells per dimension in every replica
+        self.can_tile = all(min(self._grid_of(r)) >= 3 for r in range(self.R))
+        if family is None and tiled is not None:
+            family = "rows" if tiled else "global"
+        if family not in (None,) + FAMILIES:
+            raise ValueError(f"family must be one of {FAMILIES}")
+        self._family_auto = family is None
+        self.family = family or "rows"
+        if not self.can_tile:
+            self.family = "global"
+        self.cap_rows = 0     # atoms per stencil column (row kernels)
+        self.cap_cell = 0     # atoms per 27-cell stencil (cell kernels)
         # stencil populations only change at rebuilds; overflow is checked
         self.cap_margin = 1.05
         self.force_field = None
@@ -172,22 +189,100 @@ class ReplicaEngine:
         self.cell_rep_np = np.repeat(np.arange(R, dtype=np.int32), caps)
 
     @property
+    def tiled(self):
+        return self.family != "global"
+
+    @property
+    def cap(self):
+        """LDS capacity of the current family (0 for the global list)."""
+        return {"rows": self.cap_rows, "cell": self.cap_cell}.get(self.family, 0)
+
+    @cap.setter
+    def cap(self, v):
+        if self.family == "rows":
+            self.cap_rows = int(v)
+        elif self.family == "cell":
+            self.cap_cell = int(v)
+
+    @property
     def use_tiled(self):
-        return self.tiled and self.cap > 0
+        return self.family != "global" and self.cap > 0
 
     def pick_cap(self):
-        """LDS capacity of the cell-tiled kernels from the largest 27-cell
-        stencil population of the last build (host read, setup only); a
-        stencil that outgrows it later is read from global memory (slower,
-        still exact)."""
-        if not self.tiled or self.cap:
+        """LDS capacities from the populations of the last build (host read,
+        setup only): per stencil column for the row kernels, per 27-cell
+        stencil for the cell kernels. A column or stencil that outgrows its
+        capacity later is read from global memory (slower, still exact); a
+        family without a capacity falls back to the global list."""
+        if not self.can_tile or self.cap_rows or self.cap_cell:
             return
-        smax = int(self.stencil_max.item())
-        need = int(smax * self.cap_margin) + 16
-        caps = (512, 1024) if self._tiled_auto else (512, 1024, 1280, 1536, 2048)
-        self.cap = next((c for c in caps if c >= need), 0)
+        cmax, smax = (int(v) for v in self.stencil_max.cpu())
+        self.cap_rows = next((c for c in ROW_CAPS if c >= int(cmax * self.cap_margin) + 8), 0)
+        cell_caps = CELL_CAPS[:2] if self._family_auto else CELL_CAPS
+        self.cap_cell = next((c for c in cell_caps if c >= int(smax * self.cap_margin) + 16), 0)
         if self.cap == 0:
-            self.tiled = False
+            self.family = "global"
+
+    def autotune(self, reps=5):
+        """Measured choice of the pair-kernel family (automatic mode only):
+        every available family (cell, rows, global list) evaluates the forces
+        of the current configuration with a small random jitter (restored
+        afterwards), each evaluation timed with HIP events after the caches were
+        flushed by a 512 MB write (as the other kernels of a step flush them),
+        and the fastest is kept. Cached per (force
+        field, replica count, atom count, list radius, LDS capacities)."""
+        if not (self._family_auto and self.can_tile and self.force_field is not None):
+            return
+        ff = self.force_field
+        key = (getattr(ff, "name", type(ff).__name__), self.R, self.N, self.rl, self.cap_rows,
+               self.cap_cell)
+        choice = _TUNE_CACHE.get(key)
+        if choice is None:
+            virial = self.has_barostat
+            cands = [f for f, c in (("cell", self.cap_cell), ("rows", self.cap_rows),
+                                    ("global", 1)) if c]
+            flush = torch.empty(128 << 20, dtype=torch.float32, device=self.device)
+            # time on thermally disordered positions (a perfect lattice repeats
+            # the same pair distances and table entries): +-0.1 A, within the
+            # skin, restored afterwards
+            saved = self.pos4[:, :3].clone()
+            g = torch.Generator(device=self.device)
+            g.manual_seed(7)
+            self.pos4[:, :3] += (torch.rand(saved.shape, generator=g, device=self.device,
+                                            dtype=torch.float64) - 0.5) * 0.2
+            times = {}
+            for fam in cands:
+                self.family = fam
+                self.compute_forces(virial=virial)
+                total = 0.0
+                for _ in range(reps):
+                    flush.fill_(1.0)
+                    a = torch.cuda.Event(enable_timing=True)
+                    b = torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    self.compute_forces(virial=virial)
+                    b.record()
+                    b.synchronize()
+                    total += a.elapsed_time(b)
+                times[fam] = total / reps
+            del flush
+            self.pos4[:, :3] = saved
+            choice = min(times, key=times.get)
+            _TUNE_CACHE[key] = choice
+            self.profile["autotune_ms"] = times
+        self.family = choice
+
+    def cell_args(self):
+        """Leading arguments of the sim_*_cell entry points."""
+        return (self.ncells, ptr(self.reps), ptr(self.cell_rep), ptr(self.cell_start),
+                ptr(self.cell_count), ptr(self.pos4), ptr(self.nbr16), ptr(self.nnbr),
+                self.max_nbr, self.cap)
+
+    def row_args(self):
+        """Leading arguments of the sim_*_rows entry points."""
+        return (self.ncells, ptr(self.reps), ptr(self.cell_rep), ptr(self.row_base), self.R,
+                ptr(self.cell_start), ptr(self.cell_count), ptr(self.pos4), ptr(self.nbr16),
+                ptr(self.nnbr), self.max_nbr, self.cap)
 
     def _grid_of(self, r):
         w = perpendicular_widths(self.specs[r].box)
@@ -305,7 +400,7 @@ class ReplicaEngine:
             "vel_half": torch.zeros((N, 3), dtype=torch.float32, device=dev),
         }
         self.nbr16 = torch.zeros((N, self.max_nbr), dtype=torch.int16, device=dev)
-        self.stencil_max = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.stencil_max = torch.zeros(2, dtype=torch.int32, device=dev)
         self.partial_cell = torch.zeros((self.ncells, _lib.SIM_NPART), dtype=torch.float64,
                                         device=dev)
         self.rep_cells = torch.as_tensor(np.array(self.cell_base, np.int32), device=dev)
@@ -320,6 +415,8 @@ class ReplicaEngine:
                                      self._ro_tmp["cell"].data_ptr())
         ro.nbr16 = self.nbr16.data_ptr()
         ro.stencil_max = self.stencil_max.data_ptr()
+        self.row_base = torch.zeros(R + 1, dtype=torch.int32, device=dev)
+        ro.row_base = self.row_base.data_ptr()
         ro.vel_half, ro.vel_half_tmp = (self.vel_half.data_ptr(),
                                         self._ro_tmp["vel_half"].data_ptr())
         self._ro = ro
@@ -406,6 +503,7 @@ class ReplicaEngine:
         """Initial list and forces at the current positions (md_model.py:91-102)."""
         self.build_list(force=True)
         self.pick_cap()
+        self.autotune()
         self.compute_forces(virial=True)
         self.reduce(0, 8)
         self._ke_now()
@@ -474,6 +572,9 @@ class ReplicaEngine:
         if ov & 1:
             raise RuntimeError(f"neighbor list overflow: an atom has more than {self.max_nbr} "
                                "neighbors; construct the engine with a larger max_nbr")
+        if ov & 8:
+            raise RuntimeError("a stencil column holds more than 16383 atoms; construct the "
+                               "engine with tiled=False")
         if ov & 4:
             raise RuntimeError("a replica's cell grid fell below 3 cells per dimension; "
                                "construct the engine with tiled=False")
@@ -562,6 +663,10 @@ class ReplicaEngine:
         lexicographic) and float64 distances, per replica indices are global."""
         return half_list_from_full(self.nbr, self.nnbr, self.max_nbr, self.reps[:ctypes_size()],
                                    self.positions(torch.float64))
+
+
+# autotune choices: key -> kernel family
+_TUNE_CACHE = {}
 
 
 def ctypes_size():

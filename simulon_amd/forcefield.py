@@ -37,10 +37,13 @@ class LJForceField:
         self.rc = float(rc)
 
     def compute(self, eng, virial=True):
+        if eng.use_tiled and eng.family == "rows":
+            call("sim_lj_force_rows", *eng.row_args(), len(eng.masses_amu), self.params,
+                 ptr(eng.force), ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow),
+                 stream_ptr())
+            return
         if eng.use_tiled:
-            call("sim_lj_force_cell", eng.ncells, ptr(eng.reps), ptr(eng.cell_rep),
-                 ptr(eng.cell_start), ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16),
-                 ptr(eng.nnbr), eng.max_nbr, eng.cap, self.params, ptr(eng.force),
+            call("sim_lj_force_cell", *eng.cell_args(), self.params, ptr(eng.force),
                  ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow), stream_ptr())
             return
         call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
@@ -90,12 +93,11 @@ class EAMForceField:
         force = eng.force if force is None else force
         partial = eng.partial if partial is None else partial
         if own and eng.use_tiled and self.params.ntypes == 1:
-            common = (eng.ncells, ptr(eng.reps), ptr(eng.cell_rep), ptr(eng.cell_start),
-                      ptr(eng.cell_count), ptr(eng.pos4), ptr(eng.nbr16), ptr(eng.nnbr),
-                      eng.max_nbr, eng.cap, self.params)
-            call("sim_eam_density_cell", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
+            fam = "rows" if eng.family == "rows" else "cell"
+            common = (*(eng.row_args() if fam == "rows" else eng.cell_args()), self.params)
+            call(f"sim_eam_density_{fam}", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
                  ptr(eng.overflow), s)
-            call("sim_eam_force_cell", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
+            call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
                  int(bool(virial)), ptr(eng.overflow), s)
             return
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),

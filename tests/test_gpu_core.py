@@ -268,27 +268,32 @@ def test_tiled_matches_global_list_kernels(ff, w_setfl):
         mk = lambda: EAMForceField(w_setfl)
         mass, rc = lat.W_MASS, 5.5
     out = {}
-    for tiled in (True, False):
+    for fam in ("cell", "rows", "global"):
         eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H)], [mass], cutoff=rc, skin=1.0,
-                            tiled=tiled)
+                            family=fam)
         eng.force_field = mk()
         eng.setup()
-        assert eng.use_tiled == tiled
-        out[tiled] = (eng.to_caller(eng.force).cpu().numpy(), eng.thermo_host()[0])
-    f_t, th_t = out[True]
-    f_g, th_g = out[False]
-    err = force_error(f_t, f_g)
-    print(f"{ff}: tiled vs global-list force diff {err:.2e}", flush=True)
-    assert err < 2e-6
-    assert abs(th_t[0] - th_g[0]) <= 1e-7 * abs(th_g[0])
-    np.testing.assert_allclose(th_t[2:8], th_g[2:8], rtol=0, atol=2e-6 * np.abs(th_g[2:8]).max())
+        assert eng.family == fam and eng.use_tiled == (fam != "global")
+        out[fam] = (eng.to_caller(eng.force).cpu().numpy(), eng.thermo_host()[0])
+    f_g, th_g = out["global"]
+    for fam in ("cell", "rows"):
+        f_t, th_t = out[fam]
+        err = force_error(f_t, f_g)
+        print(f"{ff}: {fam} vs global-list force diff {err:.2e}", flush=True)
+        assert err < 2e-6
+        assert abs(th_t[0] - th_g[0]) <= 1e-7 * abs(th_g[0])
+        np.testing.assert_allclose(th_t[2:8], th_g[2:8], rtol=0,
+                                   atol=2e-6 * np.abs(th_g[2:8]).max())
+    # cell and row kernels visit each list in the same order: equal bit for bit
+    assert np.array_equal(out["cell"][0], out["rows"][0])
 
 
+@pytest.mark.parametrize("fam", ["cell", "rows"])
 @pytest.mark.parametrize("ff", ["lj", "eam"])
-def test_tiled_stencil_overflow_reads_global_memory(ff, w_setfl):
-    """With an LDS tile too small for the stencils, every cell takes the
-    global-memory path through the same stencil table: same forces and sums,
-    bit for bit, and the overflow is reported but not raised."""
+def test_tiled_stencil_overflow_reads_global_memory(ff, fam, w_setfl):
+    """With an LDS column capacity too small for the stencil columns, every
+    cell takes the global-memory path through the same column table: same
+    forces and sums, bit for bit, and the overflow is reported but not raised."""
     if ff == "lj":
         pos, H = lat.lj_argon_npt_tri(ncell=8, seed=9)
         mk = lambda: LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
@@ -300,13 +305,14 @@ def test_tiled_stencil_overflow_reads_global_memory(ff, w_setfl):
     out = []
     for small in (False, True):
         eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H)], [mass], cutoff=rc, skin=1.0,
-                            tiled=True)
+                            family=fam)
         eng.force_field = mk()
         eng.build_list(force=True)
         eng.pick_cap()
-        assert eng.use_tiled and int(eng.stencil_max.item()) > 512
+        small_cap = 192 if fam == "rows" else 512
+        assert eng.use_tiled and eng.cap > small_cap
         if small:
-            eng.cap = 512
+            eng.cap = small_cap
         eng.compute_forces(virial=True)
         eng.reduce(0, 8)
         ov = int(eng.overflow.item())
@@ -324,18 +330,20 @@ def test_rebuild_mid_step_keeps_atoms_consistent():
     tiled path follows the global-list path."""
     pos, H = lat.lj_argon_nvt(seed=12)
     runs = {}
-    for tiled in (True, False):
+    for fam in ("cell", "rows", "global"):
         eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H, dt=0.004, ensemble="NVE",
                                          t_init=300.0)], [lat.AR_MASS], cutoff=8.5, skin=0.3,
-                            tiled=tiled, velocity_seed=5)
+                            family=fam, velocity_seed=5)
         eng.force_field = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, mode="shift")
         eng.setup()
         eng.run(60, graph=True)
         th = eng.thermo_host()[0]
-        runs[tiled] = (th[0] + th[8], int(eng.nbuild.cpu()[0]),
-                       eng.to_caller(eng.positions(unwrap=True)).cpu().numpy())
-    (e_t, nb_t, x_t), (e_g, nb_g, x_g) = runs[True], runs[False]
-    print(f"E tiled {e_t:.6f} global {e_g:.6f} rebuilds {nb_t}/{nb_g}", flush=True)
-    assert nb_t >= 5
-    assert abs(e_t - e_g) < 1e-4 * abs(e_g)
-    assert np.abs(x_t - x_g).max() < 1e-3
+        runs[fam] = (th[0] + th[8], int(eng.nbuild.cpu()[0]),
+                     eng.to_caller(eng.positions(unwrap=True)).cpu().numpy())
+    e_g, nb_g, x_g = runs["global"]
+    for fam in ("cell", "rows"):
+        e_t, nb_t, x_t = runs[fam]
+        print(f"E {fam} {e_t:.6f} global {e_g:.6f} rebuilds {nb_t}/{nb_g}", flush=True)
+        assert nb_t >= 5
+        assert abs(e_t - e_g) < 1e-4 * abs(e_g)
+        assert np.abs(x_t - x_g).max() < 1e-3
