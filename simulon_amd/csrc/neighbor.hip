@@ -300,7 +300,14 @@ __device__ __forceinline__ int stencil(int nc, int* off) {
     return 3;
 }
 
-template <typename TC, typename T, int STRIDE>
+// WRAPPED: positions were just wrapped into the primary cell and permuted
+// into cell order (sim_nl_build with a reorder), so cell_atoms is the identity
+// and the image of a candidate is its stencil cell's lattice translation. A
+// candidate clearly inside or outside the list radius (relative margin on d^2
+// 1e-9 in float64, 1e-5 in float32) is then decided from that displacement;
+// only near-ties take the reference's minimum-image arithmetic, so the pair
+// set is unchanged.
+template <typename TC, typename T, int STRIDE, bool WRAPPED>
 __global__ void __launch_bounds__(SIM_TILE)
     k_list(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
            const int* atom_cell, const int* cell_count, const int* cell_start,
@@ -325,26 +332,47 @@ __global__ void __launch_bounds__(SIM_TILE)
     // of the atom within that column's 9 cells, the cells taken in (z, y)
     // order (the visiting order below), so a column is one contiguous LDS run.
     int coff[3] = {0, 0, 0};
+    const bool fast = WRAPPED && sx == 3 && sy == 3 && sz == 3;
+    const double margin = sizeof(TC) == 8 ? 1e-9 : 1e-5;   // far above each type's rounding
+    const TC lo2 = rl2 * (TC)(1.0 - margin), hi2 = rl2 * (TC)(1.0 + margin);
     for (int a = 0; a < sz; ++a) {
         int zz = cz + oz[a];
-        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        const int fz = zz < 0 ? -1 : (zz >= nz ? 1 : 0);
+        zz -= fz * nz;
         for (int b = 0; b < sy; ++b) {
             int yy = cy + oy[b];
-            yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+            const int fy = yy < 0 ? -1 : (yy >= ny ? 1 : 0);
+            yy -= fy * ny;
             for (int c = 0; c < sx; ++c) {
                 int xx = cx + ox[c];
-                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                const int fx = xx < 0 ? -1 : (xx >= nx ? 1 : 0);
+                xx -= fx * nx;
+                // lattice translation of this stencil cell's atoms
+                const TC shx = (TC)(fx * P.h[0] + fy * P.h[3] + fz * P.h[6]);
+                const TC shy = (TC)(fx * P.h[1] + fy * P.h[4] + fz * P.h[7]);
+                const TC shz = (TC)(fx * P.h[2] + fy * P.h[5] + fz * P.h[8]);
                 const int cell = P.cell_base + (zz * ny + yy) * nx + xx;
                 const int s0 = cell_start[cell], s1 = s0 + cell_count[cell];
                 for (int s = s0; s < s1; ++s) {
-                    const int j = cell_atoms[s];
+                    const int j = WRAPPED ? s : cell_atoms[s];
                     if (j == i) continue;
-                    TC dx = xi - (TC)x[(size_t)j * STRIDE + 0];
-                    TC dy = yi - (TC)x[(size_t)j * STRIDE + 1];
-                    TC dz = zi - (TC)x[(size_t)j * STRIDE + 2];
-                    min_image_ref<TC>(dx, dy, dz, P);
-                    const TC d2 = (dx * dx + dy * dy) + dz * dz;
-                    if (d2 < rl2) {
+                    const TC xj = (TC)x[(size_t)j * STRIDE + 0];
+                    const TC yj = (TC)x[(size_t)j * STRIDE + 1];
+                    const TC zj = (TC)x[(size_t)j * STRIDE + 2];
+                    bool in;
+                    const TC ex = xi - (xj + shx), ey = yi - (yj + shy), ez = zi - (zj + shz);
+                    const TC e2 = (ex * ex + ey * ey) + ez * ez;
+                    if (fast && e2 < lo2) {
+                        in = true;
+                    } else if (fast && e2 > hi2) {
+                        in = false;
+                    } else {
+                        TC dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                        min_image_ref<TC>(dx, dy, dz, P);
+                        const TC d2 = (dx * dx + dy * dy) + dz * dz;
+                        in = d2 < rl2;
+                    }
+                    if (in) {
                         if (cnt < max_nbr) {
                             nbr[(size_t)cnt * n_stride + i] = j;
                             if (nbr16 != nullptr)
@@ -411,10 +439,16 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                            ro->vel, ro->force, ro->order, reinterpret_cast<int4*>(ro->image),
                            atom_cell, cell_atoms, ro->slot_of, ro->vel_half_tmp, ro->vel_half);
     }
-    hipLaunchKernelGGL((k_list<TC, T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles,
-                       x, flag, atom_cell, cell_count, cell_start, cell_atoms, nbr, nnbr, n_stride,
-                       max_nbr, overflow, pos_last, ro ? ro->nbr16 : nullptr,
-                       ro ? ro->stencil_max : nullptr);
+    if (ro != nullptr && tiles != nullptr)   // positions wrapped and in cell order
+        hipLaunchKernelGGL((k_list<TC, T, STRIDE, true>), dim3(ntiles), dim3(SIM_TILE), 0, st,
+                           reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,
+                           nbr, nnbr, n_stride, max_nbr, overflow, pos_last, ro->nbr16,
+                           ro->stencil_max);
+    else
+        hipLaunchKernelGGL((k_list<TC, T, STRIDE, false>), dim3(ntiles), dim3(SIM_TILE), 0, st,
+                           reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,
+                           nbr, nnbr, n_stride, max_nbr, overflow, pos_last,
+                           ro ? ro->nbr16 : nullptr, ro ? ro->stencil_max : nullptr);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
