@@ -451,6 +451,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first copy a device buffer of this many MiB five times (a kernel "
+                         "of known traffic, to scale hardware byte counters)")
     ap.add_argument("--mlp-frames", type=int, default=2000)
     ap.add_argument("--mlp-epochs", type=int, default=200)
     ap.add_argument("--mlp-threshold", type=float, default=0.1)
@@ -462,6 +465,16 @@ def main():
         raise SystemExit(2)
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb << 18                    # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 copies of {args.copy_probe_mb} MiB (read {args.copy_probe_mb} MiB + "
+            f"write {args.copy_probe_mb} MiB each)")
+        del src, dst
     if args.workload == "w_eam_replicas":
         out = bench_replicas(args)
     elif args.workload == "mlp_w":
