@@ -96,14 +96,17 @@ def eam_kernel_roofline(eng):
             call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
                  1, ptr(eng.overflow), stream_ptr())
     else:
+        rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
+        eng.quantize()
+
         def p1():
             call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
-                 ptr(eng.pos4), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(rho), ptr(fp),
-                 ptr(eng.partial), stream_ptr())
+                 ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(rho), ptr(fp),
+                 ptr(rec), ptr(eng.partial), stream_ptr())
 
         def p2():
-            call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-                 ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(fp), ptr(eng.force),
+            call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
+                 ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force),
                  ptr(eng.partial), 1, stream_ptr())
 
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
@@ -124,8 +127,16 @@ def lj_kernel_roofline(eng):
     ff = eng.force_field
     nbar = mean_full_list(eng)
 
-    def k():
-        ff.compute(eng, True)
+    if eng.use_tiled:
+        def k():
+            ff.compute(eng, True)
+    else:
+        eng.quantize()
+
+        def k():
+            call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
+                 ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force), ptr(eng.partial),
+                 1, stream_ptr())
 
     t = kernel_time_ms(k)
     b = eng.N * (12 + 4 * nbar + 12)
@@ -207,7 +218,8 @@ def bench_replicas(args):
     specs = replica_specs(args.replicas)
     natoms = np.array([len(s.positions) for s in specs])
     log(f"replicas {len(specs)} atoms total {natoms.sum()} min {natoms.min()} max {natoms.max()}")
-    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024)
+    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024,
+                        family=args.family)
     eng.force_field = EAMForceField(setfl, device=eng.device)
     eng.setup()
     eng.run(args.warmup, graph=True, virial=False)
@@ -451,6 +463,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--family", default=None, choices=["cell", "rows", "global"],
+                    help="pair-kernel family of the batched engine (default: measured choice)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy a device buffer of this many MiB five times (a kernel "
                          "of known traffic, to scale hardware byte counters)")

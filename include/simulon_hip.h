@@ -20,6 +20,8 @@
  *   nbr    int32 [max_nbr][N]  full Verlet list, column-major (entry k of atom i
  *                              at nbr[k*N + i]), global atom indices
  *   nnbr   int32 [N]
+ *   posq   int32 [N][4]  fractional fixed-point records (sim_pos_quantize) read
+ *                        by the full-list force kernels
  *
  * Reference interfaces replaced (JinZongxiao/Simulon):
  *   sim_nl_build / sim_nl_half_*   simulon_cuda.neighbor_search_cuda
@@ -32,6 +34,8 @@
  *   sim_eam_density / sim_eam_force   simulon_cuda.density_pass / force_pass
  *        ("cuda source/eam_cuda_ext.cpp:30-60") and the rest of
  *        core/force/eam_force_cu.py:461-579
+ *   sim_pos_quantize               Box.wrap_positions / minimum_image
+ *        (core/box.py:109-142) in fractional fixed point
  *   sim_integrate_first / _second  core/integrator/integrator.py:91-124
  *   sim_reduce                     the torch .sum()/einsum reductions of the
  *        force fields (e.g. core/force/lennard_jones_force.py:92-94),
@@ -107,7 +111,7 @@ typedef struct sim_lj_params {
     float rc;        /* cutoff */
     float rs;        /* switch start (mode 2) */
     int mode;        /* 0 raw, 1 force-shift, 2 polynomial switch */
-    int pad;
+    int ntypes;      /* atom types present, 1..SIM_MAX_TYPES (1: one uniform pair) */
 } sim_lj_params_t;
 
 typedef struct sim_eam_params {
@@ -203,23 +207,37 @@ int sim_nl_trigger(int R, void* reps, double* maxd2, int* flag, int* nbuild, int
 
 /* ---- force fields ----------------------------------------------------- */
 
-/* LJ energy/force/virial over the full list; writes force[N][3] and the
- * SIM_T_EPOT..SIM_T_W columns of partial[ntiles][16]. virial: 0 skips the
- * per-pair virial (columns written as 0). */
+/* Fractional fixed-point position records for the full-list kernels below:
+ * posq[i] = (round(s_k 2^32) mod 2^32 for k = 0..2, type), s = x H^-1 of the
+ * atom's replica wrapped into [0, 1) (core/box.py:132-142). The wrapped 32-bit
+ * difference of two records is the fractional minimum image
+ * (core/box.py:109-130); quantisation moves an atom by at most 2^-33 of each
+ * box vector. Recompute after positions or boxes change.
+ *   posq: int[N][4] */
+int sim_pos_quantize(const void* reps, const int* tiles, int ntiles, const double* pos4,
+                     int* posq, void* stream);
+
+/* LJ energy/force/virial over the full list (core/force/lennard_jones_force.py
+ * :15-95); writes force[N][3] and the SIM_T_EPOT..SIM_T_W columns of
+ * partial[ntiles][16]. virial: 0 skips the per-pair virial (columns written
+ * as 0). posq: records of sim_pos_quantize. */
 int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
-                 const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                 const int* posq, const int* nbr, const int* nnbr, int n_stride,
                  sim_lj_params_t lj, float* force, double* partial, int virial,
                  void* stream);
 
-/* EAM pass 1: rho_i, F(rho_i), F'(rho_i) (written to fp[N]) and the energy
- * column; pass 2: forces and virial columns. */
+/* EAM (core/force/eam_force_cu.py:461-579). Pass 1: rho_i, F(rho_i),
+ * F'(rho_i) (written to fp[N]) and the energy column; rec (optional, int[N][4],
+ * not aliasing posq) receives the records with F'(rho_i) in place of the
+ * type, for pass 2. Pass 2: forces and virial columns from rec (positions and
+ * F' of the neighbours) and posq (atom types, read when ntypes > 1). */
 int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
-                    const double* pos4, const int* nbr, const int* nnbr, int n_stride,
-                    sim_eam_params_t eam, float* rho, float* fp, double* partial,
+                    const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                    sim_eam_params_t eam, float* rho, float* fp, int* rec, double* partial,
                     void* stream);
 int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
-                  const double* pos4, const int* nbr, const int* nnbr, int n_stride,
-                  sim_eam_params_t eam, const float* fp, float* force, double* partial,
+                  const int* posq, const int* rec, const int* nbr, const int* nnbr,
+                  int n_stride, sim_eam_params_t eam, float* force, double* partial,
                   int virial, void* stream);
 
 /* Cell variants (one workgroup per cell slot, ncells = total cell

@@ -48,7 +48,7 @@ class LJParams(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_float * (SIM_MAX_TYPES * SIM_MAX_TYPES)),
                 ("sig", ctypes.c_float * (SIM_MAX_TYPES * SIM_MAX_TYPES)),
                 ("rc", ctypes.c_float), ("rs", ctypes.c_float), ("mode", ctypes.c_int),
-                ("pad", ctypes.c_int)]
+                ("ntypes", ctypes.c_int)]
 
 
 class EAMParams(ctypes.Structure):
@@ -99,8 +99,10 @@ SIGNATURES = {
     "sim_nl_half_fill": (_I, [_vp, _vp, _I, _I, _vp, _vp, _I, _vp, _vp, _LL, _vp, _vp]),
     "sim_nl_trigger": (_I, [_I, _vp, _vp, _vp, _vp, _I, _vp]),
     "sim_lj_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, LJParams, _vp, _vp, _I, _vp]),
-    "sim_eam_density": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _vp, _vp]),
-    "sim_eam_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _vp, _I,
+    "sim_pos_quantize": (_I, [_vp, _vp, _I, _vp, _vp, _vp]),
+    "sim_eam_density": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _vp, _vp,
+                             _vp]),
+    "sim_eam_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _I,
                            _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,

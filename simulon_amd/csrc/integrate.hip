@@ -259,7 +259,39 @@ __global__ void k_import(double4* pos, int n, const T* xyz) {
     pos[i] = x;
 }
 
+// Fractional fixed-point record of every atom: s = x H^-1 (row vectors,
+// core/box.py:132-142), wrapped into [0, 1), stored as round(s 2^32) mod 2^32;
+// w = atom type.
+__global__ void __launch_bounds__(SIM_TILE)
+    k_quantize(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+               const double4* __restrict__ pos, int4* __restrict__ q) {
+    const SimTile t = tile_of(tiles);
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const double4 x = pos[i];
+    unsigned u[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        double s = fma(x.z, P.hinv[6 + m], fma(x.y, P.hinv[3 + m], x.x * P.hinv[m]));
+        s -= floor(s);
+        u[m] = (unsigned)(unsigned long long)rint(s * SIM_Q_SCALE);   // 2^32 wraps to 0
+    }
+    q[i] = make_int4((int)u[0], (int)u[1], (int)u[2], (int)x.w);
+}
+
 }  // namespace
+
+extern "C" int sim_pos_quantize(const void* reps, const int* tiles, int ntiles, const double* pos4,
+                                int* posq, void* stream) {
+    if (!reps || !tiles || ntiles <= 0 || !pos4 || !posq) return SIM_E_ARG;
+    hipLaunchKernelGGL(k_quantize, dim3(ntiles), dim3(SIM_TILE), 0, as_stream(stream),
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles),
+                       reinterpret_cast<const double4*>(pos4), reinterpret_cast<int4*>(posq));
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
 
 extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
                                    sim_types_t types, double* pos4, const float* vel,

@@ -41,6 +41,58 @@ __device__ __forceinline__ void displacement(const double4& xi, const double4& x
     rz = (float)dz;
 }
 
+// ---- fractional fixed-point position records --------------------------------
+// The global-list kernels read every atom as one 16-byte record (int4): its
+// fractional coordinates s = x H^-1 (the row-vector convention of
+// core/box.py:127-142) wrapped into [0, 1) and stored as round(s 2^32) mod
+// 2^32, plus a 32-bit payload (the atom type; F'(rho) of the atom in the
+// record the EAM density pass writes for the force pass). A neighbour then
+// costs one 16-byte gather instead of a 32-byte float64 one.
+//
+// The wrapped 32-bit difference of two records is s_i - s_j - round(s_i - s_j)
+// (the fractional minimum image of core/box.py:109-130) exactly, away from a
+// tie at half a box vector; it is scaled by 2^-32 and mapped through H in
+// float64, then rounded once to float32. Quantisation moves an atom by at
+// most 2^-33 of each box vector (4e-8 A in a 340 A box), far below the
+// float32 rounding of the displacement that follows.
+// H scaled by 2^-32 (exactly): maps a 32-bit record difference to Cartesian.
+struct QBox {
+    double h[9];
+};
+
+__device__ __forceinline__ QBox load_qbox(const sim_replica_t& P) {
+    QBox b;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.h[k] = P.h[k] * SIM_Q_INV;
+    return b;
+}
+
+template <bool ORTHO>
+__device__ __forceinline__ void qdisp(const int4& a, const int4& b, const QBox& B, float& rx,
+                                      float& ry, float& rz) {
+    const double s0 = (double)(int)((unsigned)a.x - (unsigned)b.x);
+    const double s1 = (double)(int)((unsigned)a.y - (unsigned)b.y);
+    const double s2 = (double)(int)((unsigned)a.z - (unsigned)b.z);
+    if (ORTHO) {
+        rx = (float)(s0 * B.h[0]);
+        ry = (float)(s1 * B.h[4]);
+        rz = (float)(s2 * B.h[8]);
+    } else {
+        rx = (float)fma(s2, B.h[6], fma(s1, B.h[3], s0 * B.h[0]));
+        ry = (float)fma(s2, B.h[7], fma(s1, B.h[4], s0 * B.h[1]));
+        rz = (float)fma(s2, B.h[8], fma(s1, B.h[5], s0 * B.h[2]));
+    }
+}
+
+// Entry k of atom i's column-major list, or i itself at or past the end. The
+// load always happens (at a clamped, valid entry), so no branch surrounds it
+// and a batch of them stays in flight together.
+__device__ __forceinline__ int nbr_at(const int* __restrict__ nbr, int ns, int i, int k, int cnt) {
+    const int kk = max(min(k, cnt - 1), 0);
+    const int j = nbr[(size_t)kk * ns + i];
+    return k < cnt ? j : i;
+}
+
 __device__ __forceinline__ float sq3(float x, float y, float z) {
     return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
 }

@@ -9,7 +9,10 @@
 //           virial = 1/2 sum s r ; W = 1/2 sum (-s/(r+1e-8)) rij (x) rij
 // The reference computes rho with 2 float atomics per edge and forces with 6
 // ("cuda source/eam_cuda_ext_kernel.cu:47-52, 108-113"); here each thread owns
-// one atom of the full list, so both passes are register sums.
+// one atom of the full list, so both passes are register sums. Atoms are
+// read as 16-byte fractional fixed-point records (pair_common.h): pass 1 reads
+// (s, type) records and writes (s, F'(rho)) records, so that pass 2 gathers a
+// neighbour's position and F' in one 16-byte load.
 //
 // Tables are interleaved so that one 16-byte load yields both grid values a
 // linear interpolation needs for two functions: pass1[k] = (f(k), f(k+1),
@@ -35,21 +38,21 @@ __device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t
 
 // ---- pass 1 ---------------------------------------------------------------
 template <int NT, bool ORTHO>
-__device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const double4 xi, int cnt,
-                                             const BoxRegs& B, const double4* __restrict__ pos,
+__device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const int4 xi, int cnt,
+                                             const QBox& B, const int4* __restrict__ pos,
                                              const int* __restrict__ nbr, int ns,
                                              const sim_eam_params_t& E, float thr) {
     constexpr int U = SIM_NB_BATCH;
     const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
-    const int ti = (int)xi.w;
+    const int ti = NT == 1 ? 0 : xi.w;
     const int nt = NT == 1 ? 1 : E.ntypes;
     int k = 0;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * ns + i] : i;
+    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
     for (; k + U <= cnt; k += U) {
         int j[U];
-        double4 xj[U];
+        int4 xj[U];
         float r2[U];
         Lerp L[U];
         float4 v[U];
@@ -57,20 +60,19 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * ns + i] : i;
+        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             float rx, ry, rz;
-            displacement<ORTHO>(xi, xj[u], B, rx, ry, rz);
+            qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
             r2[u] = sq3(rx, ry, rz);
             L[u] = lerp_r(__builtin_sqrtf(r2[u]), E);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int tj = NT == 1 ? 0 : (int)xj[u].w;
+            const int tj = NT == 1 ? 0 : xj[u].w;
             v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
         }
         // masked (not branched) so that the table loads of the batch stay
@@ -84,12 +86,12 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
     }
     for (; k < cnt; ++k) {
         const int j = nbr[(size_t)k * ns + i];
-        const double4 xj = pos[j];
+        const int4 xj = pos[j];
         float rx, ry, rz;
-        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
+        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
         const float r2 = sq3(rx, ry, rz);
         if (r2 < thr) {
-            const int tj = NT == 1 ? 0 : (int)xj.w;
+            const int tj = NT == 1 ? 0 : xj.w;
             const Lerp L = lerp_r(__builtin_sqrtf(r2), E);
             const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
             rho += lerp2(v.x, v.y, L.frac);
@@ -101,20 +103,21 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
 template <int NT>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                  const double4* __restrict__ pos, const int* __restrict__ nbr,
+                  const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
-                  float* __restrict__ rho_out, float* __restrict__ fp_out,
+                  float* __restrict__ rho_out, float* __restrict__ fp_out, int4* __restrict__ rec,
                   double* __restrict__ partial, float thr) {
     __shared__ double lds[4 * 1];
-    const SimTile t = tiles[blockIdx.x];
+    const int tb = xcd_block(blockIdx.x, gridDim.x);
+    const SimTile t = tiles[tb];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
     double etot = 0.0;
     if (active) {
-        const BoxRegs B = load_box(P);
-        const double4 xi = pos[i];
-        const int ti = (int)xi.w;
+        const QBox B = load_qbox(P);
+        const int4 xi = pos[i];
+        const int ti = xi.w;
         const int cnt = nnbr[i];
         float rho = 0.f, ep = 0.f;
         if (P.ortho)
@@ -129,13 +132,15 @@ __global__ void __launch_bounds__(SIM_TILE)
         const float fr = xf - (float)idx;
         const float2 F = reinterpret_cast<const float2*>(E.emb)[(size_t)ti * E.n_rho + idx];
         const float2 dF = reinterpret_cast<const float2*>(E.demb)[(size_t)ti * E.n_rho + idx];
+        const float fpi = lerp2(dF.x, dF.y, fr);
         rho_out[i] = rho;
-        fp_out[i] = lerp2(dF.x, dF.y, fr);
+        fp_out[i] = fpi;
+        if (rec) rec[i] = make_int4(xi.x, xi.y, xi.z, __float_as_int(fpi));
         etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
     }
     double v[1] = {etot};
     block_sum<1>(v, lds);
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * SIM_NPART + SIM_T_EPOT] = v[0];
+    if (threadIdx.x == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
 }
 
 // ---- pass 2 ---------------------------------------------------------------
@@ -163,38 +168,38 @@ __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, 
     }
 }
 
+// rec: (s, F') records of pass 1; typ: (s, type) records, read for the
+// neighbour types only when there are several (NT == 0).
 template <int NT, bool VIRIAL, bool ORTHO>
-__device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi, int cnt, float fpi,
-                                           const BoxRegs& B, const double4* __restrict__ pos,
-                                           const int* __restrict__ nbr, int ns,
-                                           const float* __restrict__ fp, const sim_eam_params_t& E,
-                                           float thr) {
+__device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, int ti, int cnt,
+                                           const QBox& B, const int4* __restrict__ rec,
+                                           const int4* __restrict__ typ, const int* __restrict__ nbr,
+                                           int ns, const sim_eam_params_t& E, float thr) {
     constexpr int U = SIM_NB_BATCH;
     const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
-    const int ti = (int)xi.w;
     const int nt = NT == 1 ? 1 : E.ntypes;
+    const float fpi = __int_as_float(xi.w);
     int k = 0;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * ns + i] : i;
+    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
     for (; k + U <= cnt; k += U) {
-        int j[U];
-        double4 xj[U];
-        float rx[U], ry[U], rz[U], r2[U], r[U], fpj[U];
+        int j[U], tj[U];
+        int4 xj[U];
+        float rx[U], ry[U], rz[U], r2[U], r[U];
         Lerp L[U];
         float4 va[U], vb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * ns + i] : i;
+        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
 #pragma unroll
-        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
+        for (int u = 0; u < U; ++u) xj[u] = rec[j[u]];
 #pragma unroll
-        for (int u = 0; u < U; ++u) fpj[u] = fp[j[u]];
+        for (int u = 0; u < U; ++u) tj[u] = NT == 1 ? 0 : typ[j[u]].w;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            displacement<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
+            qdisp<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
             r2[u] = sq3(rx[u], ry[u], rz[u]);
             r[u] = __builtin_sqrtf(r2[u]);
             L[u] = lerp_r(r[u], E);
@@ -204,9 +209,8 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
             if (NT == 1) {
                 va[u] = T2[L[u].idx];
             } else {
-                const int tj = (int)xj[u].w;
-                va[u] = T2[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
-                vb[u] = T2[(size_t)(tj * nt + ti) * E.n_r + L[u].idx];
+                va[u] = T2[(size_t)(ti * nt + tj[u]) * E.n_r + L[u].idx];
+                vb[u] = T2[(size_t)(tj[u] * nt + ti) * E.n_r + L[u].idx];
             }
         }
 #pragma unroll
@@ -214,26 +218,26 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
             const float w = r2[u] < thr ? 1.0f : 0.0f;
             const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
             const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
-            const float s = w * (fpi * dda + fpj[u] * ddb + lerp2(va[u].z, va[u].w, L[u].frac));
+            const float s = w * (fpi * dda + __int_as_float(xj[u].w) * ddb +
+                                 lerp2(va[u].z, va[u].w, L[u].frac));
             eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
         }
     }
     for (; k < cnt; ++k) {
         const int j = nbr[(size_t)k * ns + i];
-        const double4 xj = pos[j];
+        const int4 xj = rec[j];
         float rx, ry, rz;
-        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
+        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
         const float r2 = sq3(rx, ry, rz);
         if (r2 < thr) {
             const float r = __builtin_sqrtf(r2);
             const Lerp L = lerp_r(r, E);
-            const float fpj = fp[j];
-            const int tj = NT == 1 ? 0 : (int)xj.w;
+            const int tj = NT == 1 ? 0 : typ[j].w;
             const float4 va = T2[(size_t)(ti * nt + tj) * E.n_r + L.idx];
             const float4 vb = NT == 1 ? va : T2[(size_t)(tj * nt + ti) * E.n_r + L.idx];
             const float dda = lerp2(va.x, va.y, L.frac);
             const float ddb = lerp2(vb.x, vb.y, L.frac);
-            const float s = fpi * dda + fpj * ddb + lerp2(va.z, va.w, L.frac);
+            const float s = fpi * dda + __int_as_float(xj.w) * ddb + lerp2(va.z, va.w, L.frac);
             eam_pair_force<VIRIAL>(a, rx, ry, rz, r, s);
         }
     }
@@ -242,25 +246,26 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const double4 xi,
 template <int NT, bool VIRIAL>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                const double4* __restrict__ pos, const int* __restrict__ nbr,
-                const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
-                const float* __restrict__ fp, float* __restrict__ force,
-                double* __restrict__ partial, float thr) {
+                const int4* __restrict__ typ, const int4* __restrict__ rec,
+                const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
+                sim_eam_params_t E, float* __restrict__ force, double* __restrict__ partial,
+                float thr) {
     __shared__ double lds[4 * 7];
-    const SimTile t = tiles[blockIdx.x];
+    const int tb = xcd_block(blockIdx.x, gridDim.x);
+    const SimTile t = tiles[tb];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
     ForceAcc a;
     if (active) {
-        const BoxRegs B = load_box(P);
-        const double4 xi = pos[i];
+        const QBox B = load_qbox(P);
+        const int4 xi = rec[i];
+        const int ti = NT == 1 ? 0 : typ[i].w;
         const int cnt = nnbr[i];
-        const float fpi = fp[i];
         if (P.ortho)
-            force_atom<NT, VIRIAL, true>(a, i, xi, cnt, fpi, B, pos, nbr, n_stride, fp, E, thr);
+            force_atom<NT, VIRIAL, true>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E, thr);
         else
-            force_atom<NT, VIRIAL, false>(a, i, xi, cnt, fpi, B, pos, nbr, n_stride, fp, E, thr);
+            force_atom<NT, VIRIAL, false>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E, thr);
         force[(size_t)i * 3 + 0] = a.fx;
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
@@ -269,7 +274,7 @@ __global__ void __launch_bounds__(SIM_TILE)
                    0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
     block_sum<7>(v, lds);
     if (threadIdx.x == 0) {
-        double* p = partial + (size_t)blockIdx.x * SIM_NPART;
+        double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
     }
@@ -278,58 +283,60 @@ __global__ void __launch_bounds__(SIM_TILE)
 }  // namespace
 
 extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
-                               const double* pos4, const int* nbr, const int* nnbr, int n_stride,
-                               sim_eam_params_t eam, float* rho, float* fp, double* partial,
-                               void* stream) {
+                               const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                               sim_eam_params_t eam, float* rho, float* fp, int* rec,
+                               double* partial, void* stream) {
     (void)R;
-    if (!reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 || !rho ||
+    if (!reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 || !rho ||
         !fp || !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
-        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES)
+        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (const int*)rec == posq)
         return SIM_E_ARG;
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
-    auto* X = reinterpret_cast<const double4*>(pos4);
+    auto* X = reinterpret_cast<const int4*>(posq);
+    auto* W = reinterpret_cast<int4*>(rec);
     hipStream_t st = as_stream(stream);
     if (eam.ntypes == 1)
         hipLaunchKernelGGL((k_eam_density<1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
-                           nnbr, n_stride, eam, rho, fp, partial, thr);
+                           nnbr, n_stride, eam, rho, fp, W, partial, thr);
     else
         hipLaunchKernelGGL((k_eam_density<0>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
-                           nnbr, n_stride, eam, rho, fp, partial, thr);
+                           nnbr, n_stride, eam, rho, fp, W, partial, thr);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
 
 extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
-                             const double* pos4, const int* nbr, const int* nnbr, int n_stride,
-                             sim_eam_params_t eam, const float* fp, float* force, double* partial,
+                             const int* posq, const int* rec, const int* nbr, const int* nnbr,
+                             int n_stride, sim_eam_params_t eam, float* force, double* partial,
                              int virial, void* stream) {
     (void)R;
-    if (!reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 || !fp ||
+    if (!reps || !tiles || ntiles <= 0 || !posq || !rec || !nbr || !nnbr || n_stride <= 0 ||
         !force || !partial || !eam.pass2 || eam.n_r < 2 || eam.ntypes < 1 ||
         eam.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
-    auto* X = reinterpret_cast<const double4*>(pos4);
+    auto* X = reinterpret_cast<const int4*>(posq);
+    auto* Q = reinterpret_cast<const int4*>(rec);
     hipStream_t st = as_stream(stream);
     const dim3 g(ntiles), b(SIM_TILE);
     if (eam.ntypes == 1) {
         if (virial)
-            hipLaunchKernelGGL((k_eam_force<1, true>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
-                               eam, fp, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<1, true>), g, b, 0, st, P, T, X, Q, nbr, nnbr, n_stride,
+                               eam, force, partial, thr);
         else
-            hipLaunchKernelGGL((k_eam_force<1, false>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
-                               eam, fp, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<1, false>), g, b, 0, st, P, T, X, Q, nbr, nnbr,
+                               n_stride, eam, force, partial, thr);
     } else {
         if (virial)
-            hipLaunchKernelGGL((k_eam_force<0, true>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
-                               eam, fp, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<0, true>), g, b, 0, st, P, T, X, Q, nbr, nnbr, n_stride,
+                               eam, force, partial, thr);
         else
-            hipLaunchKernelGGL((k_eam_force<0, false>), g, b, 0, st, P, T, X, nbr, nnbr, n_stride,
-                               eam, fp, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<0, false>), g, b, 0, st, P, T, X, Q, nbr, nnbr,
+                               n_stride, eam, force, partial, thr);
     }
     SIM_CHECK_LAUNCH();
     return SIM_OK;

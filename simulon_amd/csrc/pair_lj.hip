@@ -8,17 +8,20 @@
 //     atomics per edge, "cuda source/lj_energy_force_kernel.cu:109-116");
 //   * neighbors are taken SIM_NB_BATCH at a time with every index and
 //     position load of the batch in flight together;
-//   * the displacement is formed in float64 from float64 positions and
+//   * neighbours are read as 16-byte fractional fixed-point records (one
+//     gather each); the displacement is formed from them in float64 and
 //     rounded once to float32; everything after that is float32;
 //   * energy and virial are halved per atom (each pair is seen from both
 //     sides), reduced over the workgroup in float64 with wave shuffles, one
 //     partial per tile; sim_reduce adds the tiles with compensation.
+#include <type_traits>
+
 #include "pair_common.h"
 
 namespace {
 
 struct LJAcc {
-    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, vir = 0.f;
+    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
     float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
 };
 
@@ -60,7 +63,6 @@ __device__ __forceinline__ void lj_pair(LJAcc& a, float rx, float ry, float rz, 
     a.fz = __builtin_fmaf(fr, rz, a.fz);
     a.e += U;
     if (VIRIAL) {
-        a.vir = __builtin_fmaf(fr, r2, a.vir);
         const float gx = fr * rx, gy = fr * ry;
         a.wxx = __builtin_fmaf(gx, rx, a.wxx);
         a.wyy = __builtin_fmaf(gy, ry, a.wyy);
@@ -71,117 +73,141 @@ __device__ __forceinline__ void lj_pair(LJAcc& a, float rx, float ry, float rz, 
     }
 }
 
-template <bool VIRIAL, int MODE, bool ORTHO>
-__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const double4 xi, int cnt,
-                                        const BoxRegs& B, const double4* __restrict__ pos,
-                                        const int* __restrict__ nbr, int n_stride,
-                                        const sim_lj_params_t& lj, float thr) {
+// Pair parameters of atom type ti against every type, held in registers for
+// the atom's whole list (NT1: one type, uniform scalars).
+template <bool NT1>
+struct LJRow {
+    float eps[SIM_MAX_TYPES], sig[SIM_MAX_TYPES];
+    __device__ __forceinline__ LJRow(const sim_lj_params_t& lj, int ti) {
+#pragma unroll
+        for (int t = 0; t < SIM_MAX_TYPES; ++t) {
+            eps[t] = NT1 ? lj.eps[0] : lj.eps[ti * SIM_MAX_TYPES + t];
+            sig[t] = NT1 ? lj.sig[0] : lj.sig[ti * SIM_MAX_TYPES + t];
+        }
+    }
+    __device__ __forceinline__ float e(int tj) const {
+        return NT1 ? eps[0] : (tj == 0 ? eps[0] : tj == 1 ? eps[1] : tj == 2 ? eps[2] : eps[3]);
+    }
+    __device__ __forceinline__ float s(int tj) const {
+        return NT1 ? sig[0] : (tj == 0 ? sig[0] : tj == 1 ? sig[1] : tj == 2 ? sig[2] : sig[3]);
+    }
+};
+
+template <bool VIRIAL, int MODE, bool NT1, bool ORTHO>
+__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, const QBox& B,
+                                        const int4* __restrict__ pos, const int* __restrict__ nbr,
+                                        int n_stride, const sim_lj_params_t& lj, float thr) {
     constexpr int U = SIM_NB_BATCH;
-    const int ti = (int)xi.w;
+    const LJRow<NT1> row(lj, xi.w);
     int k = 0;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * n_stride + i] : i;
+    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, u, cnt);
     for (; k + U <= cnt; k += U) {
         int j[U];
-        double4 xj[U];
+        int4 xj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * n_stride + i] : i;
+        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, k + U + u, cnt);
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             float rx, ry, rz;
-            displacement<ORTHO>(xi, xj[u], B, rx, ry, rz);
-            const int tj = (int)xj[u].w;
-            lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr,
-                                  lj.eps[ti * SIM_MAX_TYPES + tj], lj.sig[ti * SIM_MAX_TYPES + tj],
-                                  lj);
+            qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
+            lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, row.e(xj[u].w),
+                                  row.s(xj[u].w), lj);
         }
     }
     for (; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * n_stride + i];
-        const double4 xj = pos[j];
+        const int4 xj = pos[nbr[(size_t)k * n_stride + i]];
         float rx, ry, rz;
-        displacement<ORTHO>(xi, xj, B, rx, ry, rz);
-        const int tj = (int)xj.w;
-        lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, lj.eps[ti * SIM_MAX_TYPES + tj],
-                              lj.sig[ti * SIM_MAX_TYPES + tj], lj);
+        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
+        lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, row.e(xj.w), row.s(xj.w), lj);
     }
 }
 
-template <bool VIRIAL, int MODE>
+template <bool VIRIAL, int MODE, bool NT1>
 __global__ void __launch_bounds__(SIM_TILE)
     k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-         const double4* __restrict__ pos, const int* __restrict__ nbr,
+         const int4* __restrict__ pos, const int* __restrict__ nbr,
          const int* __restrict__ nnbr, int n_stride, sim_lj_params_t lj,
          float* __restrict__ force, double* __restrict__ partial, float thr) {
     __shared__ double lds[4 * 8];
-    const SimTile t = tiles[blockIdx.x];
+    const int tb = xcd_block(blockIdx.x, gridDim.x);
+    const SimTile t = tiles[tb];
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
     LJAcc a;
     if (active) {
-        const BoxRegs B = load_box(P);
-        const double4 xi = pos[i];
+        const QBox B = load_qbox(P);
+        const int4 xi = pos[i];
         const int cnt = nnbr[i];
         if (P.ortho)
-            lj_atom<VIRIAL, MODE, true>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
         else
-            lj_atom<VIRIAL, MODE, false>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
         force[(size_t)i * 3 + 0] = a.fx;
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
     }
-    double v[8] = {0.5 * a.e, 0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
-                   0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
+    double v[8] = {0.5 * a.e, 0.5 * (a.wxx + a.wyy + a.wzz), 0.5 * a.wxx, 0.5 * a.wyy,
+                   0.5 * a.wzz, 0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
     block_sum<8>(v, lds);
     if (threadIdx.x == 0) {
-        double* p = partial + (size_t)blockIdx.x * SIM_NPART;
+        double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 8; ++k) p[k] = v[k];
     }
 }
 
-template <bool V>
+template <bool V, bool NT1>
 void launch_mode(int mode, dim3 g, hipStream_t st, const sim_replica_t* reps, const SimTile* tiles,
-                 const double4* pos, const int* nbr, const int* nnbr, int ns, sim_lj_params_t lj,
+                 const int4* pos, const int* nbr, const int* nnbr, int ns, sim_lj_params_t lj,
                  float* f, double* part, float thr) {
     if (mode == 1)
-        hipLaunchKernelGGL((k_lj<V, 1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 1, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_lj<V, 2>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 2, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr);
     else
-        hipLaunchKernelGGL((k_lj<V, 0>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 0, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr);
 }
 
 }  // namespace
 
 extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
-                            const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                            const int* posq, const int* nbr, const int* nnbr, int n_stride,
                             sim_lj_params_t lj, float* force, double* partial, int virial,
                             void* stream) {
-    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 ||
-        !force || !partial || lj.mode < 0 || lj.mode > 2 || !(lj.rc > 0.f))
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
+        !force || !partial || lj.mode < 0 || lj.mode > 2 || !(lj.rc > 0.f) || lj.ntypes < 1 ||
+        lj.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;
     const float thr = sqrt_threshold(lj.rc);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
-    auto* X = reinterpret_cast<const double4*>(pos4);
-    if (virial)
-        launch_mode<true>(lj.mode, dim3(ntiles), as_stream(stream), P, T, X, nbr, nnbr, n_stride, lj,
-                          force, partial, thr);
-    else
-        launch_mode<false>(lj.mode, dim3(ntiles), as_stream(stream), P, T, X, nbr, nnbr, n_stride,
-                           lj, force, partial, thr);
+    auto* X = reinterpret_cast<const int4*>(posq);
+    const dim3 g(ntiles);
+    hipStream_t st = as_stream(stream);
+    auto go = [&](auto v, auto one) {
+        launch_mode<decltype(v)::value, decltype(one)::value>(lj.mode, g, st, P, T, X, nbr, nnbr,
+                                                              n_stride, lj, force, partial, thr);
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
+    if (lj.ntypes == 1) {
+        if (virial) go(yes(), yes());
+        else go(no(), yes());
+    } else {
+        if (virial) go(yes(), no());
+        else go(no(), no());
+    }
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

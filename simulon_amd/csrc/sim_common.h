@@ -11,6 +11,21 @@
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Scale of the fractional fixed-point position records (sim_pos_quantize).
+constexpr double SIM_Q_SCALE = 4294967296.0;                // 2^32
+constexpr double SIM_Q_INV = 2.3283064365386962890625e-10;  // 2^-32
+
+// gfx950 dispatches workgroup b of a launch to XCD b % 8, and each XCD has
+// its own L2. Tiles are in cell order, so neighbouring tiles share most of
+// their neighbour positions: give each XCD a contiguous run of tiles
+// (q or q + 1 of them) instead of every eighth one. Bijective on [0, nb).
+#define SIM_NXCD 8
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb / SIM_NXCD, r = nb % SIM_NXCD;
+    const int x = b % SIM_NXCD, k = b / SIM_NXCD;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Tile table entry: (replica, first global atom index).
 struct SimTile {
     int rep;

@@ -362,6 +362,7 @@ class ReplicaEngine:
         self.vel = torch.from_numpy(vel).to(dev)
         self.vel_half = torch.zeros_like(self.vel)
         self.force = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        self.posq = torch.zeros((N, 4), dtype=torch.int32, device=dev)
         self.tiles = torch.from_numpy(self.tiles_np).to(dev)
         self.rep_tiles = torch.from_numpy(self.rep_tiles_np).to(dev)
         self.reps = _lib.struct_tensor([self._replica_struct(r) for r in range(R)], dev)
@@ -448,6 +449,12 @@ class ReplicaEngine:
              ptr(self.atom_cell), ptr(self.atom_slot), ptr(self.cell_count), ptr(self.cell_start),
              ptr(self.cell_atoms), ptr(self.nbr), ptr(self.nnbr), self.max_nbr,
              ptr(self.overflow), int(self.precise_list), ctypes.byref(self._ro), s)
+
+    def quantize(self):
+        """Fractional fixed-point records of the current positions (read by
+        the full-list force kernels)."""
+        call("sim_pos_quantize", ptr(self.reps), ptr(self.tiles), self.ntiles, ptr(self.pos4),
+             ptr(self.posq), stream_ptr())
 
     def compute_forces(self, virial=True):
         if self.force_field is None:

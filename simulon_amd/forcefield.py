@@ -33,10 +33,14 @@ class LJForceField:
         p.rc = float(np.float32(rc))
         p.rs = float(np.float32(switch_ratio * rc))
         p.mode = LJ_MODES[mode] if isinstance(mode, str) else int(mode)
+        p.ntypes = nt
         self.params = p
+        self.ntypes = nt
         self.rc = float(rc)
 
     def compute(self, eng, virial=True):
+        if len(eng.masses_amu) > self.ntypes:
+            raise ValueError(f"{len(eng.masses_amu)} atom types, LJ tables for {self.ntypes}")
         if eng.use_tiled and eng.family == "rows":
             call("sim_lj_force_rows", *eng.row_args(), len(eng.masses_amu), self.params,
                  ptr(eng.force), ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow),
@@ -46,7 +50,8 @@ class LJForceField:
             call("sim_lj_force_cell", *eng.cell_args(), self.params, ptr(eng.force),
                  ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow), stream_ptr())
             return
-        call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        eng.quantize()
+        call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(eng.force), ptr(eng.partial),
              int(bool(virial)), stream_ptr())
 
@@ -100,8 +105,11 @@ class EAMForceField:
             call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
                  int(bool(virial)), ptr(eng.overflow), s)
             return
-        call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp), ptr(partial), s)
-        call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(fp), ptr(force), ptr(partial),
+        rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
+        eng.quantize()
+        call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp), ptr(rec),
+             ptr(partial), s)
+        call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
+             ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(force), ptr(partial),
              int(bool(virial)), s)
