@@ -130,9 +130,16 @@ __global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2,
 // Per-replica sums of partial columns [c0, c1) in two levels: workgroup
 // (r, k) sums chunk k of replica r's rows with a compensated (Neumaier)
 // running sum per column and a fixed float64 tree across its 256 threads;
-// one workgroup per replica then adds the SIM_RED_CHUNKS chunk sums in order.
-// Both levels have a fixed order, so the result is deterministic.
+// one workgroup per replica then adds the chunk sums in order. A chunk holds
+// at least 256 rows (one per thread), so a small replica uses few chunks and
+// the other workgroups of its column return at once. Both levels have a fixed
+// order, so the result is deterministic.
 #define SIM_RED_CHUNKS 64
+
+// Rows per chunk of a replica with n rows, and the number of chunks used.
+__device__ __forceinline__ int red_len(int n) {
+    return max(256, (n + SIM_RED_CHUNKS - 1) / SIM_RED_CHUNKS);
+}
 
 template <int NC>
 __global__ void __launch_bounds__(256)
@@ -141,8 +148,9 @@ __global__ void __launch_bounds__(256)
     __shared__ double lds[4 * NC];
     const int r = blockIdx.y, ck = blockIdx.x;
     const int b0 = rep_rows[r], e0 = rep_rows[r + 1];
-    const int len = (e0 - b0 + SIM_RED_CHUNKS - 1) / SIM_RED_CHUNKS;
+    const int len = red_len(e0 - b0);
     const int b = b0 + ck * len, e = min(e0, b + len);
+    if (ck > 0 && b >= e) return;   // unused chunk (chunk 0 always writes)
     double s[NC], comp[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) s[c] = comp[c] = 0.0;
@@ -168,12 +176,14 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void k_reduce_final(int c0, int c1, const double* __restrict__ chunks,
-                               double* __restrict__ thermo) {
+__global__ void k_reduce_final(const int* __restrict__ rep_rows, int c0, int c1,
+                               const double* __restrict__ chunks, double* __restrict__ thermo) {
     const int r = blockIdx.x, c = c0 + (int)threadIdx.x;
     if (c >= c1) return;
+    const int n = rep_rows[r + 1] - rep_rows[r];
+    const int nck = max(1, (n + red_len(n) - 1) / red_len(n));
     double s = 0.0, comp = 0.0;
-    for (int k = 0; k < SIM_RED_CHUNKS; ++k) {
+    for (int k = 0; k < nck; ++k) {
         const double x = chunks[((size_t)r * SIM_RED_CHUNKS + k) * SIM_NPART + c];
         const double tt = s + x;
         comp += fabs(s) >= fabs(x) ? (s - tt) + x : (x - tt) + s;
@@ -358,7 +368,8 @@ extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int
     else
         hipLaunchKernelGGL(k_reduce_chunks<SIM_NPART>, g, dim3(256), 0, st, rep_rows, partial,
                            col0, col1, chunks);
-    hipLaunchKernelGGL(k_reduce_final, dim3(R), dim3(SIM_NPART), 0, st, col0, col1, chunks, thermo);
+    hipLaunchKernelGGL(k_reduce_final, dim3(R), dim3(SIM_NPART), 0, st, rep_rows, col0, col1,
+                       chunks, thermo);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

@@ -175,19 +175,37 @@ __global__ void __launch_bounds__(1024) k_scan(const sim_replica_t* reps, const 
 }
 
 // row_base[r] = number of cell rows (ny * nz) of replicas before r; rows of
-// cells along x are the work items of the row pair kernels.
-__global__ void k_row_base(int R, const sim_replica_t* reps, const int* flag, int* row_base) {
+// cells along x are the work items of the row pair kernels. One workgroup of
+// 256 threads (block-wide scan in chunks of 256 replicas).
+__global__ void __launch_bounds__(256)
+    k_row_base(int R, const sim_replica_t* reps, const int* flag, int* row_base) {
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (flag != nullptr) {
-        bool any = false;
-        for (int r = 0; r < R; ++r) any = any || flag[r] != 0;
-        if (!any) return;
+        int any = 0;
+        for (int r = tid; r < R; r += 256) any |= flag[r] != 0;
+        if (!__syncthreads_or(any)) return;
     }
-    int acc = 0;
-    for (int r = 0; r < R; ++r) {
-        row_base[r] = acc;
-        acc += reps[r].ncell[1] * reps[r].ncell[2];
+    int carry = 0;
+    for (int base = 0; base < R; base += 256) {
+        const int r = base + tid;
+        const int v = r < R ? reps[r].ncell[1] * reps[r].ncell[2] : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wid; ++w) before += wsum[w];
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (r < R) row_base[r] = carry + before + incl - v;
+        carry += total;
+        __syncthreads();
     }
-    row_base[R] = acc;
+    if (tid == 0) row_base[R] = carry;
 }
 
 __global__ void k_scatter(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
@@ -201,21 +219,23 @@ __global__ void k_scatter(const sim_replica_t* reps, const SimTile* tiles, const
     cell_atoms[cell_start[atom_cell[i]] + atom_slot[i]] = i;
 }
 
-__global__ void k_cell_order(const int* cell_rep, int ncells, const int* flag,
-                             const int* cell_count, const int* cell_start, int* cell_atoms) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells || !rep_active(flag, cell_rep ? cell_rep[c] : 0)) return;
-    int* a = cell_atoms + cell_start[c];
+// Rank of each atom among the atoms of its cell by index: the binning placed
+// them in arrival order (atomics); scattering again by rank gives every cell
+// its atoms in ascending index order (deterministic lists).
+__global__ void k_cell_rank(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
+                            const int* atom_cell, const int* cell_start, const int* cell_count,
+                            const int* cell_atoms, int* atom_slot) {
+    const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const int c = atom_cell[i];
+    const int* a = cell_atoms + cell_start[c];
     const int n = cell_count[c];
-    for (int k = 1; k < n; ++k) {
-        int v = a[k];
-        int m = k - 1;
-        while (m >= 0 && a[m] > v) {
-            a[m + 1] = a[m];
-            --m;
-        }
-        a[m + 1] = v;
-    }
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += a[k] < i ? 1 : 0;
+    atom_slot[i] = rank;
 }
 
 // Physical reorder of every rebuilding replica into cell order, wrapping each
@@ -415,11 +435,13 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                        flag, cell_count, atom_cell, atom_slot);
     hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag);
     if (ro != nullptr && ro->row_base != nullptr)
-        hipLaunchKernelGGL(k_row_base, dim3(1), dim3(1), 0, st, R, reps, flag, ro->row_base);
+        hipLaunchKernelGGL(k_row_base, dim3(1), dim3(256), 0, st, R, reps, flag, ro->row_base);
     hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                        atom_cell, atom_slot, cell_start, cell_atoms);
-    hipLaunchKernelGGL(k_cell_order, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
-                       ncells, flag, cell_count, cell_start, cell_atoms);
+    hipLaunchKernelGGL(k_cell_rank, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                       atom_cell, cell_start, cell_count, cell_atoms, atom_slot);
+    hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                       atom_cell, atom_slot, cell_start, cell_atoms);
     if (ro != nullptr && tiles != nullptr) {
         if (!ro->vel || !ro->force || !ro->order || !ro->slot_of || !ro->image || !ro->pos_tmp ||
             !ro->vel_tmp || !ro->force_tmp || !ro->order_tmp || !ro->image_tmp || !ro->cell_tmp ||

@@ -113,3 +113,9 @@ static inline float sqrt_threshold(float rc) {
 // Neighbors processed per batch: all loads of a batch are issued before any
 // of them is used, so their latencies overlap.
 #define SIM_NB_BATCH 4
+// Batch of the full-list kernels (thread per atom): the last batch of a list
+// is padded with masked self entries instead of a serial tail. Measured on
+// gfx950, 16 in flight per thread beats a higher occupancy with fewer.
+#ifndef SIM_LIST_BATCH
+#define SIM_LIST_BATCH 16
+#endif

@@ -37,66 +37,69 @@ __device__ __forceinline__ Lerp lerp_r(float r, const sim_eam_params_t& E) {
 __device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t * (b - a); }
 
 // ---- pass 1 ---------------------------------------------------------------
+// One batch of UB list entries starting at k (indices in j[]). MASKED: the
+// batch may run past the end of the list; those entries are self (r = 0) and
+// masked out.
+template <int UB, bool MASKED, int NT, bool ORTHO>
+__device__ __forceinline__ void density_batch(float& rho, float& ep, const int (&j)[UB], int k,
+                                              int cnt, const int4 xi, int ti, const QBox& B,
+                                              const int4* __restrict__ pos,
+                                              const sim_eam_params_t& E, float thr) {
+    const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
+    const int nt = NT == 1 ? 1 : E.ntypes;
+    int4 xj[UB];
+    float r2[UB];
+    Lerp L[UB];
+    float4 v[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        float rx, ry, rz;
+        qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
+        r2[u] = sq3(rx, ry, rz);
+        L[u] = lerp_r(__builtin_sqrtf(r2[u]), E);
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        const int tj = NT == 1 ? 0 : xj[u].w;
+        v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
+    }
+    // masked (not branched) so that the table loads of the batch stay
+    // unconditional and in flight together; + 0.0f leaves sums unchanged
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        const float w = ((!MASKED || k + u < cnt) && r2[u] < thr) ? 1.0f : 0.0f;
+        rho += w * lerp2(v[u].x, v[u].y, L[u].frac);
+        ep += w * lerp2(v[u].z, v[u].w, L[u].frac);
+    }
+}
+
 template <int NT, bool ORTHO>
 __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const int4 xi, int cnt,
                                              const QBox& B, const int4* __restrict__ pos,
                                              const int* __restrict__ nbr, int ns,
                                              const sim_eam_params_t& E, float thr) {
-    constexpr int U = SIM_NB_BATCH;
-    const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
+    constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
     const int ti = NT == 1 ? 0 : xi.w;
-    const int nt = NT == 1 ? 1 : E.ntypes;
     int k = 0;
     int jn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
     for (; k + U <= cnt; k += U) {
         int j[U];
-        int4 xj[U];
-        float r2[U];
-        Lerp L[U];
-        float4 v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-#pragma unroll
-        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            float rx, ry, rz;
-            qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
-            r2[u] = sq3(rx, ry, rz);
-            L[u] = lerp_r(__builtin_sqrtf(r2[u]), E);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int tj = NT == 1 ? 0 : xj[u].w;
-            v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
-        }
-        // masked (not branched) so that the table loads of the batch stay
-        // unconditional and in flight together; + 0.0f leaves sums unchanged
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const float w = r2[u] < thr ? 1.0f : 0.0f;
-            rho += w * lerp2(v[u].x, v[u].y, L[u].frac);
-            ep += w * lerp2(v[u].z, v[u].w, L[u].frac);
-        }
+        density_batch<U, false, NT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, thr);
     }
-    for (; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * ns + i];
-        const int4 xj = pos[j];
-        float rx, ry, rz;
-        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
-        const float r2 = sq3(rx, ry, rz);
-        if (r2 < thr) {
-            const int tj = NT == 1 ? 0 : xj.w;
-            const Lerp L = lerp_r(__builtin_sqrtf(r2), E);
-            const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
-            rho += lerp2(v.x, v.y, L.frac);
-            ep += lerp2(v.z, v.w, L.frac);
-        }
+    for (; k < cnt; k += U2) {   // the rest in small masked batches
+        int j[U2];
+#pragma unroll
+        for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
+        density_batch<U2, true, NT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, thr);
     }
 }
 
@@ -168,6 +171,50 @@ __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, 
     }
 }
 
+template <int UB, bool MASKED, int NT, bool VIRIAL, bool ORTHO>
+__device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int k, int cnt,
+                                            const int4 xi, int ti, float fpi, const QBox& B,
+                                            const int4* __restrict__ rec,
+                                            const int4* __restrict__ typ,
+                                            const sim_eam_params_t& E, float thr) {
+    const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
+    const int nt = NT == 1 ? 1 : E.ntypes;
+    int tj[UB];
+    int4 xj[UB];
+    float rx[UB], ry[UB], rz[UB], r2[UB], r[UB];
+    Lerp L[UB];
+    float4 va[UB], vb[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) xj[u] = rec[j[u]];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) tj[u] = NT == 1 ? 0 : typ[j[u]].w;
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        qdisp<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
+        r2[u] = sq3(rx[u], ry[u], rz[u]);
+        r[u] = __builtin_sqrtf(r2[u]);
+        L[u] = lerp_r(r[u], E);
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        if (NT == 1) {
+            va[u] = T2[L[u].idx];
+        } else {
+            va[u] = T2[(size_t)(ti * nt + tj[u]) * E.n_r + L[u].idx];
+            vb[u] = T2[(size_t)(tj[u] * nt + ti) * E.n_r + L[u].idx];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        const float w = ((!MASKED || k + u < cnt) && r2[u] < thr) ? 1.0f : 0.0f;
+        const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
+        const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
+        const float s = w * (fpi * dda + __int_as_float(xj[u].w) * ddb +
+                             lerp2(va[u].z, va[u].w, L[u].frac));
+        eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
+    }
+}
+
 // rec: (s, F') records of pass 1; typ: (s, type) records, read for the
 // neighbour types only when there are several (NT == 0).
 template <int NT, bool VIRIAL, bool ORTHO>
@@ -175,71 +222,25 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
                                            const QBox& B, const int4* __restrict__ rec,
                                            const int4* __restrict__ typ, const int* __restrict__ nbr,
                                            int ns, const sim_eam_params_t& E, float thr) {
-    constexpr int U = SIM_NB_BATCH;
-    const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
-    const int nt = NT == 1 ? 1 : E.ntypes;
+    constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
     const float fpi = __int_as_float(xi.w);
     int k = 0;
     int jn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
     for (; k + U <= cnt; k += U) {
-        int j[U], tj[U];
-        int4 xj[U];
-        float rx[U], ry[U], rz[U], r2[U], r[U];
-        Lerp L[U];
-        float4 va[U], vb[U];
+        int j[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-#pragma unroll
-        for (int u = 0; u < U; ++u) xj[u] = rec[j[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u) tj[u] = NT == 1 ? 0 : typ[j[u]].w;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            qdisp<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
-            r2[u] = sq3(rx[u], ry[u], rz[u]);
-            r[u] = __builtin_sqrtf(r2[u]);
-            L[u] = lerp_r(r[u], E);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (NT == 1) {
-                va[u] = T2[L[u].idx];
-            } else {
-                va[u] = T2[(size_t)(ti * nt + tj[u]) * E.n_r + L[u].idx];
-                vb[u] = T2[(size_t)(tj[u] * nt + ti) * E.n_r + L[u].idx];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const float w = r2[u] < thr ? 1.0f : 0.0f;
-            const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
-            const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
-            const float s = w * (fpi * dda + __int_as_float(xj[u].w) * ddb +
-                                 lerp2(va[u].z, va[u].w, L[u].frac));
-            eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
-        }
+        force_batch<U, false, NT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E, thr);
     }
-    for (; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * ns + i];
-        const int4 xj = rec[j];
-        float rx, ry, rz;
-        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
-        const float r2 = sq3(rx, ry, rz);
-        if (r2 < thr) {
-            const float r = __builtin_sqrtf(r2);
-            const Lerp L = lerp_r(r, E);
-            const int tj = NT == 1 ? 0 : typ[j].w;
-            const float4 va = T2[(size_t)(ti * nt + tj) * E.n_r + L.idx];
-            const float4 vb = NT == 1 ? va : T2[(size_t)(tj * nt + ti) * E.n_r + L.idx];
-            const float dda = lerp2(va.x, va.y, L.frac);
-            const float ddb = lerp2(vb.x, vb.y, L.frac);
-            const float s = fpi * dda + __int_as_float(xj.w) * ddb + lerp2(va.z, va.w, L.frac);
-            eam_pair_force<VIRIAL>(a, rx, ry, rz, r, s);
-        }
+    for (; k < cnt; k += U2) {   // the rest in small masked batches
+        int j[U2];
+#pragma unroll
+        for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
+        force_batch<U2, true, NT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E, thr);
     }
 }
 

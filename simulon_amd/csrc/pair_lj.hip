@@ -6,7 +6,7 @@
 //     (coalesced index loads), so the per-atom force is a register sum — no
 //     atomics, deterministic order (the reference's kernel issues 6 float
 //     atomics per edge, "cuda source/lj_energy_force_kernel.cu:109-116");
-//   * neighbors are taken SIM_NB_BATCH at a time with every index and
+//   * neighbors are taken SIM_LJ_BATCH at a time with every index and
 //     position load of the batch in flight together;
 //   * neighbours are read as 16-byte fractional fixed-point records (one
 //     gather each); the displacement is formed from them in float64 and
@@ -93,39 +93,48 @@ struct LJRow {
     }
 };
 
+// One batch of UB list entries from k (indices in j[]); MASKED: entries past
+// the end of the list (self) fail the cutoff test.
+template <int UB, bool MASKED, bool VIRIAL, int MODE, bool NT1, bool ORTHO>
+__device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, int cnt,
+                                         const int4 xi, const LJRow<NT1>& row, const QBox& B,
+                                         const int4* __restrict__ pos, const sim_lj_params_t& lj,
+                                         float thr) {
+    int4 xj[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        float rx, ry, rz;
+        qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
+        const float r2 = (!MASKED || k + u < cnt) ? sq3(rx, ry, rz) : thr;
+        lj_pair<VIRIAL, MODE>(a, rx, ry, rz, r2, thr, row.e(xj[u].w), row.s(xj[u].w), lj);
+    }
+}
+
+// LJ takes the whole list in masked batches of SIM_LJ_BATCH (measured faster
+// than a large unmasked batch plus a small masked rest, unlike EAM).
+#ifndef SIM_LJ_BATCH
+#define SIM_LJ_BATCH 8
+#endif
+
 template <bool VIRIAL, int MODE, bool NT1, bool ORTHO>
 __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, const QBox& B,
                                         const int4* __restrict__ pos, const int* __restrict__ nbr,
                                         int n_stride, const sim_lj_params_t& lj, float thr) {
-    constexpr int U = SIM_NB_BATCH;
+    constexpr int U = SIM_LJ_BATCH;
     const LJRow<NT1> row(lj, xi.w);
-    int k = 0;
     int jn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, u, cnt);
-    for (; k + U <= cnt; k += U) {
+    for (int k = 0; k < cnt; k += U) {
         int j[U];
-        int4 xj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, k + U + u, cnt);
-#pragma unroll
-        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            float rx, ry, rz;
-            qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
-            lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, row.e(xj[u].w),
-                                  row.s(xj[u].w), lj);
-        }
-    }
-    for (; k < cnt; ++k) {
-        const int4 xj = pos[nbr[(size_t)k * n_stride + i]];
-        float rx, ry, rz;
-        qdisp<ORTHO>(xi, xj, B, rx, ry, rz);
-        lj_pair<VIRIAL, MODE>(a, rx, ry, rz, sq3(rx, ry, rz), thr, row.e(xj.w), row.s(xj.w), lj);
+        lj_batch<U, true, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
     }
 }
 
