@@ -373,31 +373,49 @@ __global__ void __launch_bounds__(SIM_TILE)
                 const TC shz = (TC)(fx * P.h[2] + fy * P.h[5] + fz * P.h[8]);
                 const int cell = P.cell_base + (zz * ny + yy) * nx + xx;
                 const int s0 = cell_start[cell], s1 = s0 + cell_count[cell];
-                for (int s = s0; s < s1; ++s) {
-                    const int j = WRAPPED ? s : cell_atoms[s];
-                    if (j == i) continue;
-                    const TC xj = (TC)x[(size_t)j * STRIDE + 0];
-                    const TC yj = (TC)x[(size_t)j * STRIDE + 1];
-                    const TC zj = (TC)x[(size_t)j * STRIDE + 2];
-                    bool in;
-                    const TC ex = xi - (xj + shx), ey = yi - (yj + shy), ez = zi - (zj + shz);
-                    const TC e2 = (ex * ex + ey * ey) + ez * ez;
-                    if (fast && e2 < lo2) {
-                        in = true;
-                    } else if (fast && e2 > hi2) {
-                        in = false;
-                    } else {
-                        TC dx = xi - xj, dy = yi - yj, dz = zi - zj;
-                        min_image_ref<TC>(dx, dy, dz, P);
-                        const TC d2 = (dx * dx + dy * dy) + dz * dz;
-                        in = d2 < rl2;
+                // candidates in batches of 8: every load of a batch in flight
+                // together, the tests into a bit mask, then the appends in order
+                constexpr int B = 8;
+                for (int sb = s0; sb < s1; sb += B) {
+                    TC px[B], py[B], pz[B];
+                    int jb[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        const int ss = min(sb + u, s1 - 1);
+                        jb[u] = WRAPPED ? ss : cell_atoms[ss];
+                        px[u] = (TC)x[(size_t)jb[u] * STRIDE + 0];
+                        py[u] = (TC)x[(size_t)jb[u] * STRIDE + 1];
+                        pz[u] = (TC)x[(size_t)jb[u] * STRIDE + 2];
                     }
-                    if (in) {
+                    unsigned m = 0;
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        if (sb + u >= s1 || jb[u] == i) continue;
+                        const TC xj = px[u], yj = py[u], zj = pz[u];
+                        bool in;
+                        const TC ex = xi - (xj + shx), ey = yi - (yj + shy), ez = zi - (zj + shz);
+                        const TC e2 = (ex * ex + ey * ey) + ez * ez;
+                        if (fast && e2 < lo2) {
+                            in = true;
+                        } else if (fast && e2 > hi2) {
+                            in = false;
+                        } else {
+                            TC dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                            min_image_ref<TC>(dx, dy, dz, P);
+                            const TC d2 = (dx * dx + dy * dy) + dz * dz;
+                            in = d2 < rl2;
+                        }
+                        m |= in ? 1u << u : 0u;
+                    }
+                    while (m != 0) {
+                        const int u = __builtin_ctz(m);
+                        m &= m - 1;
+                        const int ss = sb + u;
                         if (cnt < max_nbr) {
-                            nbr[(size_t)cnt * n_stride + i] = j;
+                            nbr[(size_t)cnt * n_stride + i] = WRAPPED ? ss : cell_atoms[ss];
                             if (nbr16 != nullptr)
                                 nbr16[(size_t)i * max_nbr + cnt] =
-                                    (unsigned short)((c << 14) | (coff[c] + (s - s0)));
+                                    (unsigned short)((c << 14) | (coff[c] + (ss - s0)));
                         }
                         ++cnt;
                     }

@@ -18,6 +18,8 @@
 // linear interpolation needs for two functions: pass1[k] = (f(k), f(k+1),
 // phi(k), phi(k+1)), pass2[k] = (f'(k), f'(k+1), phi'(k), phi'(k+1)), with
 // the k+1 entry clamped at the last grid point as the reference clamps `next`.
+#include <type_traits>
+
 #include "pair_common.h"
 
 namespace {
@@ -36,15 +38,62 @@ __device__ __forceinline__ Lerp lerp_r(float r, const sim_eam_params_t& E) {
 
 __device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t * (b - a); }
 
+// Grid point k of a table pair (both values a linear interpolation needs, of
+// two functions): the interleaved float4 table in global memory, or (LT) the
+// compact copy in LDS, tl[k] = (first function, second function) at k, one
+// extra point past the end repeating the last (the clamp of the interleaved
+// table).
+template <bool LT>
+__device__ __forceinline__ float4 tab_get(const float4* __restrict__ g, const float2* tl,
+                                          size_t k) {
+    if (LT) {
+        const float2 a = tl[k], b = tl[k + 1];
+        return make_float4(a.x, b.x, a.y, b.y);
+    }
+    return g[k];
+}
+
+// Fill the LDS copy of an interleaved table of n points (n + 1 entries).
+__device__ __forceinline__ void tab_fill(float2* tl, const float4* __restrict__ g, int n) {
+    for (int k = threadIdx.x; k <= n; k += blockDim.x) {
+        const float4 e = g[min(k, n - 1)];
+        tl[k] = make_float2(e.x, e.z);
+    }
+    __syncthreads();
+}
+
+// Tile walk of one workgroup. Persistent (LT): the workgroups on XCD x
+// (blockIdx.x % 8 == x, grid a multiple of 8) take XCD x's contiguous run of
+// tiles round robin; otherwise one tile per workgroup (xcd_block).
+struct TileWalk {
+    int base, m, end, step;
+    __device__ __forceinline__ TileWalk(bool persistent, int ntiles) {
+        if (persistent) {
+            const int x = blockIdx.x % SIM_NXCD;
+            const int q = ntiles / SIM_NXCD, r = ntiles % SIM_NXCD;
+            base = x * q + min(x, r);
+            end = q + (x < r ? 1 : 0);
+            m = blockIdx.x / SIM_NXCD;
+            step = gridDim.x / SIM_NXCD;
+        } else {
+            base = xcd_block(blockIdx.x, gridDim.x);
+            m = 0;
+            end = 1;
+            step = 1;
+        }
+    }
+};
+
 // ---- pass 1 ---------------------------------------------------------------
 // One batch of UB list entries starting at k (indices in j[]). MASKED: the
 // batch may run past the end of the list; those entries are self (r = 0) and
 // masked out.
-template <int UB, bool MASKED, int NT, bool ORTHO>
+template <int UB, bool MASKED, int NT, bool LT, bool ORTHO>
 __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (&j)[UB], int k,
                                               int cnt, const int4 xi, int ti, const QBox& B,
                                               const int4* __restrict__ pos,
-                                              const sim_eam_params_t& E, float thr) {
+                                              const sim_eam_params_t& E, const float2* tl,
+                                              float thr) {
     const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
     const int nt = NT == 1 ? 1 : E.ntypes;
     int4 xj[UB];
@@ -63,7 +112,7 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         const int tj = NT == 1 ? 0 : xj[u].w;
-        v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
+        v[u] = tab_get<LT>(T1, tl, (size_t)(ti * nt + tj) * E.n_r + L[u].idx);
     }
     // masked (not branched) so that the table loads of the batch stay
     // unconditional and in flight together; + 0.0f leaves sums unchanged
@@ -75,11 +124,12 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
     }
 }
 
-template <int NT, bool ORTHO>
+template <int NT, bool LT, bool ORTHO>
 __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const int4 xi, int cnt,
                                              const QBox& B, const int4* __restrict__ pos,
                                              const int* __restrict__ nbr, int ns,
-                                             const sim_eam_params_t& E, float thr) {
+                                             const sim_eam_params_t& E, const float2* tl,
+                                             float thr) {
     constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
     const int ti = NT == 1 ? 0 : xi.w;
     int k = 0;
@@ -93,57 +143,62 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
         // indices of the next batch load while this batch computes
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-        density_batch<U, false, NT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, thr);
+        density_batch<U, false, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
     }
     for (; k < cnt; k += U2) {   // the rest in small masked batches
         int j[U2];
 #pragma unroll
         for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
-        density_batch<U2, true, NT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, thr);
+        density_batch<U2, true, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
     }
 }
 
-template <int NT>
+template <int NT, bool LT>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                  const int4* __restrict__ pos, const int* __restrict__ nbr,
+                  int ntiles, const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
                   float* __restrict__ rho_out, float* __restrict__ fp_out, int4* __restrict__ rec,
                   double* __restrict__ partial, float thr) {
+    extern __shared__ float2 tl[];
     __shared__ double lds[4 * 1];
-    const int tb = xcd_block(blockIdx.x, gridDim.x);
-    const SimTile t = tiles[tb];
-    const sim_replica_t& P = reps[t.rep];
-    const int i = t.start + threadIdx.x;
-    const bool active = i < P.atom_base + P.natoms;
-    double etot = 0.0;
-    if (active) {
-        const QBox B = load_qbox(P);
-        const int4 xi = pos[i];
-        const int ti = xi.w;
-        const int cnt = nnbr[i];
-        float rho = 0.f, ep = 0.f;
-        if (P.ortho)
-            density_atom<NT, true>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, thr);
-        else
-            density_atom<NT, false>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, thr);
-        // embedding, eam_force_cu.py:448-459
-        const float lo = E.rho_lo[ti];
-        const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
-        const float xf = (rcl - lo) * E.inv_drho[ti];
-        const int idx = (int)xf;
-        const float fr = xf - (float)idx;
-        const float2 F = reinterpret_cast<const float2*>(E.emb)[(size_t)ti * E.n_rho + idx];
-        const float2 dF = reinterpret_cast<const float2*>(E.demb)[(size_t)ti * E.n_rho + idx];
-        const float fpi = lerp2(dF.x, dF.y, fr);
-        rho_out[i] = rho;
-        fp_out[i] = fpi;
-        if (rec) rec[i] = make_int4(xi.x, xi.y, xi.z, __float_as_int(fpi));
-        etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
+    if (LT) tab_fill(tl, static_cast<const float4*>(E.pass1), E.n_r);
+    for (TileWalk w(LT, ntiles); w.m < w.end; w.m += w.step) {
+        const int tb = w.base + w.m;
+        const SimTile t = tiles[tb];
+        const sim_replica_t& P = reps[t.rep];
+        const int i = t.start + threadIdx.x;
+        const bool active = i < P.atom_base + P.natoms;
+        double etot = 0.0;
+        if (active) {
+            const QBox B = load_qbox(P);
+            const int4 xi = pos[i];
+            const int ti = xi.w;
+            const int cnt = nnbr[i];
+            float rho = 0.f, ep = 0.f;
+            if (P.ortho)
+                density_atom<NT, LT, true>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, tl, thr);
+            else
+                density_atom<NT, LT, false>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, tl, thr);
+            // embedding, eam_force_cu.py:448-459
+            const float lo = E.rho_lo[ti];
+            const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
+            const float xf = (rcl - lo) * E.inv_drho[ti];
+            const int idx = (int)xf;
+            const float fr = xf - (float)idx;
+            const float2 F = reinterpret_cast<const float2*>(E.emb)[(size_t)ti * E.n_rho + idx];
+            const float2 dF = reinterpret_cast<const float2*>(E.demb)[(size_t)ti * E.n_rho + idx];
+            const float fpi = lerp2(dF.x, dF.y, fr);
+            rho_out[i] = rho;
+            fp_out[i] = fpi;
+            if (rec) rec[i] = make_int4(xi.x, xi.y, xi.z, __float_as_int(fpi));
+            etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
+        }
+        double v[1] = {etot};
+        block_sum<1>(v, lds);
+        if (threadIdx.x == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
+        __syncthreads();   // lds is reused by the next tile
     }
-    double v[1] = {etot};
-    block_sum<1>(v, lds);
-    if (threadIdx.x == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
 }
 
 // ---- pass 2 ---------------------------------------------------------------
@@ -171,12 +226,13 @@ __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, 
     }
 }
 
-template <int UB, bool MASKED, int NT, bool VIRIAL, bool ORTHO>
+template <int UB, bool MASKED, int NT, bool LT, bool VIRIAL, bool ORTHO>
 __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int k, int cnt,
                                             const int4 xi, int ti, float fpi, const QBox& B,
                                             const int4* __restrict__ rec,
                                             const int4* __restrict__ typ,
-                                            const sim_eam_params_t& E, float thr) {
+                                            const sim_eam_params_t& E, const float2* tl,
+                                            float thr) {
     const float4* __restrict__ T2 = static_cast<const float4*>(E.pass2);
     const int nt = NT == 1 ? 1 : E.ntypes;
     int tj[UB];
@@ -198,7 +254,7 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         if (NT == 1) {
-            va[u] = T2[L[u].idx];
+            va[u] = tab_get<LT>(T2, tl, L[u].idx);
         } else {
             va[u] = T2[(size_t)(ti * nt + tj[u]) * E.n_r + L[u].idx];
             vb[u] = T2[(size_t)(tj[u] * nt + ti) * E.n_r + L[u].idx];
@@ -217,11 +273,12 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
 
 // rec: (s, F') records of pass 1; typ: (s, type) records, read for the
 // neighbour types only when there are several (NT == 0).
-template <int NT, bool VIRIAL, bool ORTHO>
+template <int NT, bool LT, bool VIRIAL, bool ORTHO>
 __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, int ti, int cnt,
                                            const QBox& B, const int4* __restrict__ rec,
                                            const int4* __restrict__ typ, const int* __restrict__ nbr,
-                                           int ns, const sim_eam_params_t& E, float thr) {
+                                           int ns, const sim_eam_params_t& E, const float2* tl,
+                                           float thr) {
     constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
     const float fpi = __int_as_float(xi.w);
     int k = 0;
@@ -234,51 +291,94 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-        force_batch<U, false, NT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E, thr);
+        force_batch<U, false, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
+                                                     tl, thr);
     }
     for (; k < cnt; k += U2) {   // the rest in small masked batches
         int j[U2];
 #pragma unroll
         for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
-        force_batch<U2, true, NT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E, thr);
+        force_batch<U2, true, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
+                                                     tl, thr);
     }
 }
 
-template <int NT, bool VIRIAL>
+template <int NT, bool LT, bool VIRIAL>
 __global__ void __launch_bounds__(SIM_TILE)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                const int4* __restrict__ typ, const int4* __restrict__ rec,
+                int ntiles, const int4* __restrict__ typ, const int4* __restrict__ rec,
                 const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
                 sim_eam_params_t E, float* __restrict__ force, double* __restrict__ partial,
                 float thr) {
+    extern __shared__ float2 tl[];
     __shared__ double lds[4 * 7];
-    const int tb = xcd_block(blockIdx.x, gridDim.x);
-    const SimTile t = tiles[tb];
-    const sim_replica_t& P = reps[t.rep];
-    const int i = t.start + threadIdx.x;
-    const bool active = i < P.atom_base + P.natoms;
-    ForceAcc a;
-    if (active) {
-        const QBox B = load_qbox(P);
-        const int4 xi = rec[i];
-        const int ti = NT == 1 ? 0 : typ[i].w;
-        const int cnt = nnbr[i];
-        if (P.ortho)
-            force_atom<NT, VIRIAL, true>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E, thr);
-        else
-            force_atom<NT, VIRIAL, false>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E, thr);
-        force[(size_t)i * 3 + 0] = a.fx;
-        force[(size_t)i * 3 + 1] = a.fy;
-        force[(size_t)i * 3 + 2] = a.fz;
-    }
-    double v[7] = {0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
-                   0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
-    block_sum<7>(v, lds);
-    if (threadIdx.x == 0) {
-        double* p = partial + (size_t)tb * SIM_NPART;
+    if (LT) tab_fill(tl, static_cast<const float4*>(E.pass2), E.n_r);
+    for (TileWalk w(LT, ntiles); w.m < w.end; w.m += w.step) {
+        const int tb = w.base + w.m;
+        const SimTile t = tiles[tb];
+        const sim_replica_t& P = reps[t.rep];
+        const int i = t.start + threadIdx.x;
+        const bool active = i < P.atom_base + P.natoms;
+        ForceAcc a;
+        if (active) {
+            const QBox B = load_qbox(P);
+            const int4 xi = rec[i];
+            const int ti = NT == 1 ? 0 : typ[i].w;
+            const int cnt = nnbr[i];
+            if (P.ortho)
+                force_atom<NT, LT, VIRIAL, true>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E,
+                                                 tl, thr);
+            else
+                force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E,
+                                                  tl, thr);
+            force[(size_t)i * 3 + 0] = a.fx;
+            force[(size_t)i * 3 + 1] = a.fy;
+            force[(size_t)i * 3 + 2] = a.fz;
+        }
+        double v[7] = {0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
+                       0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
+        block_sum<7>(v, lds);
+        if (threadIdx.x == 0) {
+            double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
+            for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
+        }
+        __syncthreads();   // lds is reused by the next tile
     }
+}
+
+// LDS-resident tables: one element type, the compact copy within
+// SIM_EAM_LDS_TAB_MAX bytes. Grid of the persistent kernels: two workgroups
+// per CU (the LDS each holds), a multiple of 8 (one run of tiles per XCD).
+#ifndef SIM_EAM_LDS_TAB
+#define SIM_EAM_LDS_TAB 1
+#endif
+#define SIM_EAM_LDS_TAB_MAX (96 * 1024)
+
+size_t lds_tab_bytes(const sim_eam_params_t& E) {
+    return (size_t)(E.n_r + 1) * sizeof(float2);
+}
+
+bool use_lds_tab(const sim_eam_params_t& E) {
+    return SIM_EAM_LDS_TAB && E.ntypes == 1 && lds_tab_bytes(E) <= SIM_EAM_LDS_TAB_MAX;
+}
+
+int persistent_grid(int ntiles) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+        ncu = 256;
+    const int g = min(ntiles, 2 * ncu);
+    return (g + SIM_NXCD - 1) / SIM_NXCD * SIM_NXCD;
+}
+
+// Raise a kernel's dynamic LDS limit to SIM_EAM_LDS_TAB_MAX (once per kernel).
+template <typename K>
+bool allow_lds(K kernel) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               SIM_EAM_LDS_TAB_MAX) == hipSuccess;
 }
 
 }  // namespace
@@ -298,12 +398,18 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     auto* X = reinterpret_cast<const int4*>(posq);
     auto* W = reinterpret_cast<int4*>(rec);
     hipStream_t st = as_stream(stream);
-    if (eam.ntypes == 1)
-        hipLaunchKernelGGL((k_eam_density<1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
-                           nnbr, n_stride, eam, rho, fp, W, partial, thr);
+    const size_t tb = lds_tab_bytes(eam);
+    static const bool lds_ok = allow_lds(k_eam_density<1, true>);
+    if (use_lds_tab(eam) && lds_ok)
+        hipLaunchKernelGGL((k_eam_density<1, true>), dim3(persistent_grid(ntiles)),
+                           dim3(SIM_TILE), tb, st, P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho,
+                           fp, W, partial, thr);
+    else if (eam.ntypes == 1)
+        hipLaunchKernelGGL((k_eam_density<1, false>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T,
+                           ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
     else
-        hipLaunchKernelGGL((k_eam_density<0>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T, X, nbr,
-                           nnbr, n_stride, eam, rho, fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<0, false>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T,
+                           ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
@@ -324,21 +430,22 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     auto* Q = reinterpret_cast<const int4*>(rec);
     hipStream_t st = as_stream(stream);
     const dim3 g(ntiles), b(SIM_TILE);
-    if (eam.ntypes == 1) {
-        if (virial)
-            hipLaunchKernelGGL((k_eam_force<1, true>), g, b, 0, st, P, T, X, Q, nbr, nnbr, n_stride,
-                               eam, force, partial, thr);
+    const size_t tb = lds_tab_bytes(eam);
+    auto go = [&](auto v) {
+        constexpr bool V = decltype(v)::value;
+        static const bool lds_ok = allow_lds(k_eam_force<1, true, V>);
+        if (use_lds_tab(eam) && lds_ok)
+            hipLaunchKernelGGL((k_eam_force<1, true, V>), dim3(persistent_grid(ntiles)), b, tb, st,
+                               P, T, ntiles, X, Q, nbr, nnbr, n_stride, eam, force, partial, thr);
+        else if (eam.ntypes == 1)
+            hipLaunchKernelGGL((k_eam_force<1, false, V>), g, b, 0, st, P, T, ntiles, X, Q, nbr,
+                               nnbr, n_stride, eam, force, partial, thr);
         else
-            hipLaunchKernelGGL((k_eam_force<1, false>), g, b, 0, st, P, T, X, Q, nbr, nnbr,
-                               n_stride, eam, force, partial, thr);
-    } else {
-        if (virial)
-            hipLaunchKernelGGL((k_eam_force<0, true>), g, b, 0, st, P, T, X, Q, nbr, nnbr, n_stride,
-                               eam, force, partial, thr);
-        else
-            hipLaunchKernelGGL((k_eam_force<0, false>), g, b, 0, st, P, T, X, Q, nbr, nnbr,
-                               n_stride, eam, force, partial, thr);
-    }
+            hipLaunchKernelGGL((k_eam_force<0, false, V>), g, b, 0, st, P, T, ntiles, X, Q, nbr,
+                               nnbr, n_stride, eam, force, partial, thr);
+    };
+    if (virial) go(std::true_type());
+    else go(std::false_type());
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
