@@ -55,6 +55,21 @@ def timed(fn, steps):
     return time.perf_counter() - t0
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)["kernels"].get(f"{workload}/{kernel}")
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else int(rec["bytes_per_launch"])
+
+
 def mean_full_list(eng):
     return float(eng.nnbr.double().mean().item())
 
@@ -73,7 +88,7 @@ def kernel_time_ms(launch, reps=50):
     return a.elapsed_time(b) / reps
 
 
-def eam_kernel_roofline(eng):
+def eam_kernel_roofline(eng, workload="w_eam_replicas"):
     """Roofline of the EAM passes with the algorithmic bytes per atom:
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
     read) + 12 (F write). The dominant (longer) pass is reported; the kernels
@@ -117,12 +132,13 @@ def eam_kernel_roofline(eng):
     name, t, b = (pre, t2, b2) if t2 >= t1 else (pre1, t1, b1)
     ach = b / (t * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload, name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
             "other_pass_ms": round(t1 if t == t2 else t2, 5)}
 
 
-def lj_kernel_roofline(eng):
+def lj_kernel_roofline(eng, workload="lj_argon_npt_tri"):
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
@@ -141,8 +157,10 @@ def lj_kernel_roofline(eng):
     t = kernel_time_ms(k)
     b = eng.N * (12 + 4 * nbar + 12)
     ach = b / (t * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": "lj_force" + (f"_{eng.family}" if eng.use_tiled else ""), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+    name = "lj_force" + (f"_{eng.family}" if eng.use_tiled else "")
+    return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload, name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2)}
 
 
@@ -313,8 +331,8 @@ def bench_single(args):
         drift = ((th[0] + th[8]) - e0) / eng.N / (2000 * dt_ps * 1e-3)
         log(f"NVE: E/atom {e0 / eng.N:.6f} -> {(th[0] + th[8]) / eng.N:.6f} eV, drift "
             f"{drift:.3e} eV/atom/ns")
-    roof = lj_kernel_roofline(eng) if isinstance(eng.force_field, LJForceField) else \
-        eam_kernel_roofline(eng)
+    roof = lj_kernel_roofline(eng, w) if isinstance(eng.force_field, LJForceField) else \
+        eam_kernel_roofline(eng, w)
     return {"metric": f"atom-steps/s ({w})", "value": round(float(rate), 1), "unit": "atom-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(t * 1e3 / args.steps, 4), "higher_is_better": True,
@@ -355,7 +373,8 @@ def mlp_kernel_roofline(eng):
     ach = b / (t * 1e-3) / 1e9
     flops = eng.N * (2 * (32 * 64 + 64 * 64 + 64) + 2 * (64 * 64 + 64 * 32) + 64)
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("mlp_w", name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
             "other_pass_ms": round(t1 if t == t2 else t2, 5),
             "energy_pass_dense_tflops": round(flops / (t1 * 1e-3) / 1e12, 2)}
@@ -473,9 +492,8 @@ def main():
     ap.add_argument("--mlp-threshold", type=float, default=0.1)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(replicas and domains shard across GPUs by design only; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (replicas and "
+              "domains would shard across GPUs; that part is design only)", file=sys.stderr)
         raise SystemExit(2)
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")

@@ -53,11 +53,19 @@ __device__ __forceinline__ float4 tab_get(const float4* __restrict__ g, const fl
     return g[k];
 }
 
-// Fill the LDS copy of an interleaved table of n points (n + 1 entries).
+// Fill the LDS copy of an interleaved table of n points (n + 1 entries), 8
+// loads per thread in flight.
 __device__ __forceinline__ void tab_fill(float2* tl, const float4* __restrict__ g, int n) {
-    for (int k = threadIdx.x; k <= n; k += blockDim.x) {
-        const float4 e = g[min(k, n - 1)];
-        tl[k] = make_float2(e.x, e.z);
+    constexpr int F = 8;
+    for (int k0 = threadIdx.x; k0 <= n; k0 += F * blockDim.x) {
+        float4 e[F];
+#pragma unroll
+        for (int u = 0; u < F; ++u) e[u] = g[min(k0 + u * (int)blockDim.x, n - 1)];
+#pragma unroll
+        for (int u = 0; u < F; ++u) {
+            const int k = k0 + u * (int)blockDim.x;
+            if (k <= n) tl[k] = make_float2(e[u].x, e[u].z);
+        }
     }
     __syncthreads();
 }
@@ -359,8 +367,13 @@ size_t lds_tab_bytes(const sim_eam_params_t& E) {
     return (size_t)(E.n_r + 1) * sizeof(float2);
 }
 
-bool use_lds_tab(const sim_eam_params_t& E) {
-    return SIM_EAM_LDS_TAB && E.ntypes == 1 && lds_tab_bytes(E) <= SIM_EAM_LDS_TAB_MAX;
+// Small launches take the global tables: filling the LDS copy costs each
+// workgroup a fixed time that only a long tile walk repays.
+#define SIM_EAM_LDS_MIN_TILES 256
+
+bool use_lds_tab(const sim_eam_params_t& E, int ntiles) {
+    return SIM_EAM_LDS_TAB && E.ntypes == 1 && lds_tab_bytes(E) <= SIM_EAM_LDS_TAB_MAX &&
+           ntiles >= SIM_EAM_LDS_MIN_TILES;
 }
 
 int persistent_grid(int ntiles) {
@@ -400,7 +413,7 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     hipStream_t st = as_stream(stream);
     const size_t tb = lds_tab_bytes(eam);
     static const bool lds_ok = allow_lds(k_eam_density<1, true>);
-    if (use_lds_tab(eam) && lds_ok)
+    if (use_lds_tab(eam, ntiles) && lds_ok)
         hipLaunchKernelGGL((k_eam_density<1, true>), dim3(persistent_grid(ntiles)),
                            dim3(SIM_TILE), tb, st, P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho,
                            fp, W, partial, thr);
@@ -434,7 +447,7 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     auto go = [&](auto v) {
         constexpr bool V = decltype(v)::value;
         static const bool lds_ok = allow_lds(k_eam_force<1, true, V>);
-        if (use_lds_tab(eam) && lds_ok)
+        if (use_lds_tab(eam, ntiles) && lds_ok)
             hipLaunchKernelGGL((k_eam_force<1, true, V>), dim3(persistent_grid(ntiles)), b, tb, st,
                                P, T, ntiles, X, Q, nbr, nnbr, n_stride, eam, force, partial, thr);
         else if (eam.ntypes == 1)

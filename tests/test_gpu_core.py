@@ -134,6 +134,37 @@ def test_eam_forces_vs_fp64(w_setfl):
     assert np.abs(W - ref["virial_tensor"]).max() <= VIR_TOL * np.abs(ref["virial_tensor"]).max()
 
 
+def test_eam_lds_tables_match_global(w_setfl):
+    """A launch of >= 256 tiles takes the persistent kernels with the pair
+    table in LDS; a small one the global tables. Same table values and
+    arithmetic: the small replica's forces, energy and virial columns agree
+    bit for bit between a packed engine (LDS path) and its own engine (global
+    path), and the big replica's forces stay within the float64 tolerance on
+    a sample of atoms."""
+    small, Hs = lat.w_bcc(8, seed=21, amp=0.08)
+    big, Hb = lat.w_bcc(40, seed=22, amp=0.08)
+    _print_case("eam_lds_tables", n_small=len(small), n_big=len(big))
+    mk = lambda p, H: ReplicaSpec(positions=p, box=H, dt=0.001)
+    eng = ReplicaEngine([mk(small, Hs), mk(big, Hb)], [lat.W_MASS], cutoff=5.5, skin=1.0,
+                        family="global")
+    eng.force_field = EAMForceField(w_setfl)
+    eng.setup()
+    assert eng.ntiles >= 256
+    e1 = ReplicaEngine([mk(small, Hs)], [lat.W_MASS], cutoff=5.5, skin=1.0, family="global")
+    e1.force_field = EAMForceField(w_setfl)
+    e1.setup()
+    assert e1.ntiles < 256
+    f_pack = eng.to_caller(eng.force).cpu().numpy()
+    np.testing.assert_array_equal(f_pack[:len(small)], e1.to_caller(e1.force).cpu().numpy())
+    np.testing.assert_array_equal(eng.thermo_host()[0][:8], e1.thermo_host()[0][:8])
+    # big replica: FP64 oracle forces on the full system, compared on all atoms
+    pos = eng.positions().cpu().numpy()[eng.replica_slice(1)]
+    ref = oracle_eam(pos, Hb, np.zeros(len(pos), int), w_setfl, 1.0)
+    err = force_error(eng.force.cpu().numpy()[eng.replica_slice(1)], ref["forces"])
+    print(f"eam LDS-table force err {err:.3e}", flush=True)
+    assert err <= FORCE_TOL
+
+
 def test_integrator_injected_noise():
     pos, H = lat.lj_argon_nvt(seed=4)
     eng = _lj_engine(pos, H, ensemble="NVT", gamma=10.0)
