@@ -279,6 +279,10 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     }
 }
 
+#ifndef SIM_EAM_FORCE_BATCH
+#define SIM_EAM_FORCE_BATCH SIM_LIST_BATCH
+#endif
+
 // rec: (s, F') records of pass 1; typ: (s, type) records, read for the
 // neighbour types only when there are several (NT == 0).
 template <int NT, bool LT, bool VIRIAL, bool ORTHO>
@@ -287,7 +291,7 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
                                            const int4* __restrict__ typ, const int* __restrict__ nbr,
                                            int ns, const sim_eam_params_t& E, const float2* tl,
                                            float thr) {
-    constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
+    constexpr int U = SIM_EAM_FORCE_BATCH, U2 = SIM_NB_BATCH;
     const float fpi = __int_as_float(xi.w);
     int k = 0;
     int jn[U];

@@ -442,6 +442,10 @@ class ReplicaEngine:
     def build_list(self, force=False):
         """Skin trigger (force=True: every replica) and the gated rebuild."""
         s = stream_ptr()
+        # the cell-row prefix is needed by the row kernels only (and at the
+        # forced build of setup, before the autotuner has chosen)
+        self._ro.row_base = (self.row_base.data_ptr()
+                             if force or (self.use_tiled and self.family == "rows") else None)
         call("sim_nl_trigger", self.R, ptr(self.reps), ptr(self.maxd2), ptr(self.flag),
              ptr(self.nbuild), int(bool(force)), s)
         call("sim_nl_build", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
@@ -479,11 +483,16 @@ class ReplicaEngine:
         attached one), kinetic columns from the integrator's."""
         s = stream_ptr()
         fa, fb = col0, min(col1, _lib.T_KE)
+        ka, kb = max(col0, _lib.T_KE), col1
+        part, rows = self.force_partials(ff) if fa < fb else (None, None)
+        if part is self.partial and rows is self.rep_tiles and ka < kb:
+            # force and kinetic columns share the per-tile rows: one pass
+            call("sim_reduce", self.R, ptr(rows), ptr(part), fa, kb, ptr(self.red_chunks),
+                 ptr(self.thermo), s)
+            return
         if fa < fb:
-            part, rows = self.force_partials(ff)
             call("sim_reduce", self.R, ptr(rows), ptr(part), fa, fb, ptr(self.red_chunks),
                  ptr(self.thermo), s)
-        ka, kb = max(col0, _lib.T_KE), col1
         if ka < kb:
             call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
                  ptr(self.red_chunks), ptr(self.thermo), s)
