@@ -40,6 +40,9 @@ from simulon_amd.engine import ReplicaEngine, ReplicaSpec  # noqa: E402
 from simulon_amd.forcefield import EAMForceField, LJForceField  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# run lengths of the configured workloads (timed steps after warm-up)
+DEFAULT_STEPS = {"w_eam_replicas": 1000, "lj_argon_nvt": 1000, "w_eam_nve": 2000,
+                 "lj_argon_npt_tri": 500, "mlp_w": 500}
 METRIC = "atom-steps/s on 1 GPU: plain vs 64-replica batched EAM; force err vs FP64"
 
 
@@ -68,6 +71,14 @@ def measured_traffic(workload, kernel):
     except (OSError, ValueError, KeyError):
         return None
     return None if rec is None else int(rec["bytes_per_launch"])
+
+
+def step_roofline(natoms, ms_per_step, bytes_per_atom_step):
+    """Whole-step algorithmic bytes per atom-step (integrator 88 B + the force
+    field's passes) over the step time, against 8 TB/s."""
+    ach = natoms * bytes_per_atom_step / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_atom_step": round(bytes_per_atom_step, 1), "achieved": round(ach, 1),
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
 def mean_full_list(eng):
@@ -283,6 +294,8 @@ def bench_replicas(args):
         "force_err_vs_fp64": ferr,
         "rebuilds_per_replica": [int(builds.min()), int(builds.max())],
         "roofline": roof,
+        "step_roofline": step_roofline(int(natoms.sum()), t * 1e3 / args.steps,
+                                       132 + 8 * roof["nbar_full"]),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_eam(setfl, seconds=args.cpu_seconds)
@@ -340,6 +353,10 @@ def bench_single(args):
             "config": {"workload": w, "atoms": eng.N},
             "ns_per_day": round(args.steps * dt_ps * 1e-3 * 86400 / t, 3),
             "rebuilds": int(eng.nbuild.cpu()[0]), "roofline": roof,
+            "step_roofline": step_roofline(
+                eng.N, t * 1e3 / args.steps,
+                (12 + 4 * roof["nbar_full"] + 12 + 88) if isinstance(eng.force_field, LJForceField)
+                else 132 + 8 * roof["nbar_full"]),
             "nve_drift_eV_per_atom_per_ns": drift}
 
 
@@ -356,13 +373,14 @@ def mlp_kernel_roofline(eng):
     c = eng.scratch("mlp_c", torch.float32, (eng.N, 32))
     part, _ = ff.partials(eng)
     st = ff.net.struct()
+    eng.quantize()
 
     def p1():
-        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), None, None, ptr(part), stream_ptr())
 
     def p2():
-        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), ptr(eng.force), ptr(part), 0,
              stream_ptr())
 
@@ -473,7 +491,8 @@ def bench_mlp(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's configured run length)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="w_eam_replicas")
     ap.add_argument("--replicas", type=int, default=64)
@@ -491,6 +510,8 @@ def main():
     ap.add_argument("--mlp-epochs", type=int, default=200)
     ap.add_argument("--mlp-threshold", type=float, default=0.1)
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload] if args.workload in DEFAULT_STEPS else 200
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (replicas and "
               "domains would shard across GPUs; that part is design only)", file=sys.stderr)

@@ -364,20 +364,21 @@ typedef struct sim_mlp_net {
 /* Descriptors and the network for every atom: per-atom dE_i/dg_i (c, float
  * [N][32], already divided by the descriptor scale), optional per-atom
  * energies e_atom [N], optional normalised descriptors gh [N][32], and the
- * energy column of partial. */
-int sim_mlp_energy(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+ * energy column of partial. posq (all MLP entry points): the fractional
+ * fixed-point records of sim_pos_quantize. */
+int sim_mlp_energy(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
                    const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net, float* c,
                    float* gh, float* e_atom, double* partial, void* stream);
 
 /* Analytic forces F_i = -sum_j (c_i + c_j) . dphi(r_ij)/dr r_ij/|r_ij| and the
  * virial columns of partial (same convention as sim_eam_force). */
-int sim_mlp_force(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+int sim_mlp_force(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
                   const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net, const float* c,
                   float* force, double* partial, int virial, void* stream);
 
 /* Force-matching adjoint: with R_i = 2 wF (F_i - Fref_i) / (3 N_replica),
  * v_i = inv_std * dL/dc_i = -inv_std * sum_j dphi(r_ij)/dr ((R_i - R_j) . r_ij/|r_ij|). */
-int sim_mlp_adjoint(int R, const void* reps, const int* tiles, int ntiles, const double* pos4,
+int sim_mlp_adjoint(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
                     const int* nbr, const int* nnbr, int n_stride, sim_mlp_net_t net,
                     const float* force, const float* force_ref, float wF, float* v, void* stream);
 

@@ -140,32 +140,34 @@ __device__ __forceinline__ void cutoff_fn(float r, const Basis& b, float& fc, fl
     dfc = -0.5f * 3.14159265358979f * b.inv_rc * s;
 }
 
-// Walk the full list of atom i in batches of SIM_NB_BATCH with every index
-// and position load of a batch in flight together; body(j, rx, ry, rz) sees
-// xi - xj at the minimum image.
+// Walk the full list of atom i in batches of SIM_MLP_BATCH with every index
+// and record load of a batch in flight together (branch-free: indices past
+// the end of the list are clamped, their entries skipped); body(j, rx, ry,
+// rz) sees xi - xj at the minimum image, from the fractional fixed-point
+// records (pair_common.h).
+#define SIM_MLP_BATCH 8
 template <bool ORTHO, typename Body>
-__device__ __forceinline__ void walk(int i, const double4 xi, int cnt, const BoxRegs& B,
-                                     const double4* __restrict__ pos, const int* __restrict__ nbr,
+__device__ __forceinline__ void walk(int i, const int4 xi, int cnt, const QBox& B,
+                                     const int4* __restrict__ pos, const int* __restrict__ nbr,
                                      int ns, Body&& body) {
-    constexpr int U = SIM_NB_BATCH;
+    constexpr int U = SIM_MLP_BATCH;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = u < cnt ? nbr[(size_t)u * ns + i] : i;
+    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
     for (int k = 0; k < cnt; k += U) {
         int j[U];
-        double4 xj[U];
+        int4 xj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            jn[u] = k + U + u < cnt ? nbr[(size_t)(k + U + u) * ns + i] : i;
+        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
 #pragma unroll
         for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (k + u < cnt) {
                 float rx, ry, rz;
-                displacement<ORTHO>(xi, xj[u], B, rx, ry, rz);
+                qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
                 body(j[u], rx, ry, rz);
             }
         }
@@ -182,11 +184,11 @@ __device__ __forceinline__ int lane_atom(const SimTile& t) {
 
 template <bool ORTHO>
 __device__ __forceinline__ void descriptors(float (&g)[8], int i, const sim_replica_t& P,
-                                            const double4* __restrict__ pos,
+                                            const int4* __restrict__ pos,
                                             const int* __restrict__ nbr,
                                             const int* __restrict__ nnbr, int ns, const Basis& b) {
-    const BoxRegs B = load_box(P);
-    const double4 xi = pos[i];
+    const QBox B = load_qbox(P);
+    const int4 xi = pos[i];
     walk<ORTHO>(i, xi, nnbr[i], B, pos, nbr, ns, [&](int, float rx, float ry, float rz) {
         const float r2 = sq3(rx, ry, rz);
         if (r2 < b.thr) {
@@ -204,7 +206,7 @@ __device__ __forceinline__ void descriptors(float (&g)[8], int i, const sim_repl
 
 __global__ void __launch_bounds__(256)
     k_mlp_energy(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                 const double4* __restrict__ pos, const int* __restrict__ nbr,
+                 const int4* __restrict__ pos, const int* __restrict__ nbr,
                  const int* __restrict__ nnbr, int ns, sim_mlp_net_t net, float* __restrict__ c,
                  float* __restrict__ gh_out, float* __restrict__ e_atom,
                  double* __restrict__ partial, float thr) {
@@ -305,12 +307,12 @@ __global__ void __launch_bounds__(256)
 
 template <bool VIRIAL, bool ORTHO>
 __device__ __forceinline__ void force_atom(float (&acc)[10], int i, const sim_replica_t& P,
-                                           const double4* __restrict__ pos,
+                                           const int4* __restrict__ pos,
                                            const int* __restrict__ nbr,
                                            const int* __restrict__ nnbr, int ns,
                                            const float* __restrict__ c, const Basis& b, int q) {
-    const BoxRegs B = load_box(P);
-    const double4 xi = pos[i];
+    const QBox B = load_qbox(P);
+    const int4 xi = pos[i];
     const f4v ci0 = *reinterpret_cast<const f4v*>(c + (size_t)i * ND + 4 * q);
     const f4v ci1 = *reinterpret_cast<const f4v*>(c + (size_t)i * ND + 16 + 4 * q);
     walk<ORTHO>(i, xi, nnbr[i], B, pos, nbr, ns, [&](int j, float rx, float ry, float rz) {
@@ -351,7 +353,7 @@ __device__ __forceinline__ void force_atom(float (&acc)[10], int i, const sim_re
 template <bool VIRIAL>
 __global__ void __launch_bounds__(256)
     k_mlp_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                const double4* __restrict__ pos, const int* __restrict__ nbr,
+                const int4* __restrict__ pos, const int* __restrict__ nbr,
                 const int* __restrict__ nnbr, int ns, sim_mlp_net_t net,
                 const float* __restrict__ c, float* __restrict__ force,
                 double* __restrict__ partial, float thr) {
@@ -398,13 +400,13 @@ __global__ void __launch_bounds__(256)
 
 template <bool ORTHO>
 __device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, const sim_replica_t& P,
-                                             const double4* __restrict__ pos,
+                                             const int4* __restrict__ pos,
                                              const int* __restrict__ nbr,
                                              const int* __restrict__ nnbr, int ns,
                                              const float* __restrict__ F,
                                              const float* __restrict__ Fr, const Basis& b) {
-    const BoxRegs B = load_box(P);
-    const double4 xi = pos[i];
+    const QBox B = load_qbox(P);
+    const int4 xi = pos[i];
     const float dix = F[(size_t)i * 3 + 0] - Fr[(size_t)i * 3 + 0];
     const float diy = F[(size_t)i * 3 + 1] - Fr[(size_t)i * 3 + 1];
     const float diz = F[(size_t)i * 3 + 2] - Fr[(size_t)i * 3 + 2];
@@ -432,7 +434,7 @@ __device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, const sim_rep
 
 __global__ void __launch_bounds__(256)
     k_mlp_adjoint(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
-                  const double4* __restrict__ pos, const int* __restrict__ nbr,
+                  const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int ns, sim_mlp_net_t net,
                   const float* __restrict__ F, const float* __restrict__ Fr, float wF,
                   float* __restrict__ vout, float thr) {
@@ -737,32 +739,32 @@ bool net_ok(const sim_mlp_net_t& n) {
 }  // namespace
 
 extern "C" int sim_mlp_energy(int R, const void* reps, const int* tiles, int ntiles,
-                              const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                              const int* posq, const int* nbr, const int* nnbr, int n_stride,
                               sim_mlp_net_t net, float* c, float* gh, float* e_atom,
                               double* partial, void* stream) {
-    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 ||
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
         !c || !partial || !net_ok(net))
         return SIM_E_ARG;
     hipLaunchKernelGGL(k_mlp_energy, dim3(ntiles * SIM_MLP_ROWS_PER_TILE), dim3(SIM_TILE), 0,
                        as_stream(stream), static_cast<const sim_replica_t*>(reps),
                        reinterpret_cast<const SimTile*>(tiles),
-                       reinterpret_cast<const double4*>(pos4), nbr, nnbr, n_stride, net, c, gh,
+                       reinterpret_cast<const int4*>(posq), nbr, nnbr, n_stride, net, c, gh,
                        e_atom, partial, sqrt_threshold(net.rc));
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
 
 extern "C" int sim_mlp_force(int R, const void* reps, const int* tiles, int ntiles,
-                             const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                             const int* posq, const int* nbr, const int* nnbr, int n_stride,
                              sim_mlp_net_t net, const float* c, float* force, double* partial,
                              int virial, void* stream) {
-    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 ||
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
         !c || !force || !partial || !net_ok(net))
         return SIM_E_ARG;
     const dim3 g(ntiles * SIM_MLP_ROWS_PER_TILE);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
-    auto* X = reinterpret_cast<const double4*>(pos4);
+    auto* X = reinterpret_cast<const int4*>(posq);
     const float thr = sqrt_threshold(net.rc);
     if (virial)
         hipLaunchKernelGGL((k_mlp_force<true>), g, dim3(SIM_TILE), 0, as_stream(stream), P, T, X,
@@ -775,16 +777,16 @@ extern "C" int sim_mlp_force(int R, const void* reps, const int* tiles, int ntil
 }
 
 extern "C" int sim_mlp_adjoint(int R, const void* reps, const int* tiles, int ntiles,
-                               const double* pos4, const int* nbr, const int* nnbr, int n_stride,
+                               const int* posq, const int* nbr, const int* nnbr, int n_stride,
                                sim_mlp_net_t net, const float* force, const float* force_ref,
                                float wF, float* v, void* stream) {
-    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !nbr || !nnbr || n_stride <= 0 ||
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
         !force || !force_ref || !v || !net_ok(net))
         return SIM_E_ARG;
     hipLaunchKernelGGL(k_mlp_adjoint, dim3(ntiles * SIM_MLP_ROWS_PER_TILE), dim3(SIM_TILE), 0,
                        as_stream(stream), static_cast<const sim_replica_t*>(reps),
                        reinterpret_cast<const SimTile*>(tiles),
-                       reinterpret_cast<const double4*>(pos4), nbr, nnbr, n_stride, net, force,
+                       reinterpret_cast<const int4*>(posq), nbr, nnbr, n_stride, net, force,
                        force_ref, wF, v, sqrt_threshold(net.rc));
     SIM_CHECK_LAUNCH();
     return SIM_OK;

@@ -161,9 +161,10 @@ class MLPForceField:
         part, _ = self.partials(eng)
         net = self.net.struct()
         s = stream_ptr()
-        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        eng.quantize()
+        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, net, ptr(c), ptr(gh), ptr(e_atom), ptr(part), s)
-        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, net, ptr(c), ptr(eng.force), ptr(part),
              int(bool(virial)), s)
 
@@ -240,13 +241,14 @@ class Trainer:
         gh = eng.scratch("mlp_gh", torch.float32, (eng.N, NDESC))
         v = eng.scratch("mlp_v", torch.float32, (eng.N, NDESC))
         part, rows = self.ff.partials(eng)
-        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        eng.quantize()
+        call("sim_mlp_energy", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), ptr(gh), None, ptr(part), s)
         call("sim_reduce", eng.R, ptr(rows), ptr(part), 0, 1, ptr(eng.red_chunks),
              ptr(eng.thermo), s)
-        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        call("sim_mlp_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(c), ptr(eng.force), ptr(part), 0, s)
-        call("sim_mlp_adjoint", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.pos4),
+        call("sim_mlp_adjoint", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, st, ptr(eng.force), ptr(frames.force_ref),
              self.wF, ptr(v), s)
         call("sim_mlp_grad", eng.N, ptr(frames.atom_rep), ptr(eng.reps), ptr(eng.thermo),
