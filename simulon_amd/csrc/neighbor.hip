@@ -320,6 +320,10 @@ __device__ __forceinline__ int stencil(int nc, int* off) {
     return 3;
 }
 
+__device__ __forceinline__ bool stencil3(const sim_replica_t& P) {
+    return P.ncell[0] >= 3 && P.ncell[1] >= 3 && P.ncell[2] >= 3;
+}
+
 // WRAPPED: positions were just wrapped into the primary cell and permuted
 // into cell order (sim_nl_build with a reorder), so cell_atoms is the identity
 // and the image of a candidate is its stencil cell's lattice translation. A
@@ -332,10 +336,12 @@ __global__ void __launch_bounds__(SIM_TILE)
     k_list(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
            const int* atom_cell, const int* cell_count, const int* cell_start,
            const int* cell_atoms, int* nbr, int* nnbr, int n_stride, int max_nbr,
-           int* overflow, double* pos_last, unsigned short* nbr16, int* stencil_max) {
+           int* overflow, double* pos_last, unsigned short* nbr16, int* stencil_max,
+           int skip_staged) {
     const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
     if (!rep_active(flag, t.rep)) return;
     const sim_replica_t& P = reps[t.rep];
+    if (skip_staged && stencil3(P)) return;   // built by k_list_staged
     const int i = t.start + threadIdx.x;
     if (i >= P.atom_base + P.natoms) return;
     const TC xi = (TC)x[(size_t)i * STRIDE + 0];
@@ -440,6 +446,252 @@ __global__ void __launch_bounds__(SIM_TILE)
     }
 }
 
+// ---- staged list build (after a reorder, replicas with >= 3 cells per
+// dimension) -------------------------------------------------------------
+// One cell at a time per workgroup: the 27 stencil cells' atoms (contiguous
+// per cell after the reorder) are staged in LDS, translated to the image next
+// to the cell, and the cell's atoms (8 lanes each, a contiguous run of the
+// staged candidates per lane) test them twice: a count pass, a prefix over
+// the 8 lanes, and a write pass. The decision per candidate and the order of
+// every list (stencil cells in (z, y, x) order, atoms in index order) are
+// k_list's (WRAPPED), so the lists are identical. A stencil of more than
+// SIM_LIST_CAP atoms reads its candidates from global memory instead.
+#define SIM_LIST_CAP 1024
+#define SIM_LIST_LANES 8
+#define SIM_LIST_GRID 2048   // persistent workgroups over the rebuilding cells
+
+// act[r] = cells before replica r of the rebuilding replicas built here;
+// act[R] = their total. One workgroup of 256 threads.
+__global__ void __launch_bounds__(256)
+    k_act_base(int R, const sim_replica_t* reps, const int* flag, int* act) {
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < R; base += 256) {
+        const int r = base + tid;
+        int v = 0;
+        if (r < R && rep_active(flag, r) && stencil3(reps[r]))
+            v = reps[r].ncell[0] * reps[r].ncell[1] * reps[r].ncell[2];
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wid; ++w) before += wsum[w];
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (r < R) act[r] = carry + before + incl - v;
+        carry += total;
+        __syncthreads();
+    }
+    if (tid == 0) act[R] = carry;
+}
+
+template <typename TC, typename T, int STRIDE>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_list_staged(int R, const sim_replica_t* reps, const int* act, const T* x,
+                  const int* cell_count, const int* cell_start, int* nbr, int* nnbr, int n_stride,
+                  int max_nbr, int* overflow, double* pos_last, unsigned short* nbr16,
+                  int* stencil_max) {
+    __shared__ TC qx[SIM_LIST_CAP], qy[SIM_LIST_CAP], qz[SIM_LIST_CAP];
+    __shared__ int qs[SIM_LIST_CAP];
+    __shared__ unsigned short q16[SIM_LIST_CAP];
+    __shared__ int mstart[27], mcount[27], mpre[28], mslot[27], mcol[3];
+    __shared__ TC msh[27][3];
+    const int tid = threadIdx.x;
+    const int total = act[R];
+    const double margin = sizeof(TC) == 8 ? 1e-9 : 1e-5;   // as k_list
+    for (int ac = blockIdx.x; ac < total; ac += gridDim.x) {
+        int lo = 0, hi = R - 1;   // last replica with act[r] <= ac
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (act[mid] <= ac) lo = mid;
+            else hi = mid - 1;
+        }
+        const sim_replica_t& P = reps[lo];
+        const int cl = ac - act[lo];
+        const int nx = P.ncell[0], ny = P.ncell[1], nz = P.ncell[2];
+        const int cx = cl % nx, cy = (cl / nx) % ny, cz = cl / (nx * ny);
+        if (tid < 27) {   // stencil cell k = (dz * 3 + dy) * 3 + dx, offsets -1..1
+            const int a = tid / 9, b = (tid / 3) % 3, c = tid % 3;
+            int zz = cz + a - 1, yy = cy + b - 1, xx = cx + c - 1;
+            const int fz = zz < 0 ? -1 : (zz >= nz ? 1 : 0);
+            const int fy = yy < 0 ? -1 : (yy >= ny ? 1 : 0);
+            const int fx = xx < 0 ? -1 : (xx >= nx ? 1 : 0);
+            zz -= fz * nz;
+            yy -= fy * ny;
+            xx -= fx * nx;
+            const int cell = P.cell_base + (zz * ny + yy) * nx + xx;
+            mstart[tid] = cell_start[cell];
+            mcount[tid] = cell_count[cell];
+            msh[tid][0] = (TC)(fx * P.h[0] + fy * P.h[3] + fz * P.h[6]);
+            msh[tid][1] = (TC)(fx * P.h[1] + fy * P.h[4] + fz * P.h[7]);
+            msh[tid][2] = (TC)(fx * P.h[2] + fy * P.h[5] + fz * P.h[8]);
+        }
+        __syncthreads();
+        if (tid == 0) {   // prefix in stencil order; column slot bases (k_list's coff)
+            int coff[3] = {0, 0, 0};
+            mpre[0] = 0;
+            for (int k = 0; k < 27; ++k) {
+                mpre[k + 1] = mpre[k] + mcount[k];
+                mslot[k] = ((k % 3) << 14) | coff[k % 3];
+                coff[k % 3] += mcount[k];
+            }
+            mcol[0] = coff[0];
+            mcol[1] = coff[1];
+            mcol[2] = coff[2];
+        }
+        __syncthreads();
+        const int nt = mpre[27];
+        const bool staged = nt <= SIM_LIST_CAP;
+        auto cell_of = [&](int f) {
+            int k = 0;
+#pragma unroll
+            for (int step = 16; step > 0; step >>= 1)
+                if (k + step <= 26 && mpre[k + step] <= f) k += step;
+            return k;
+        };
+        if (staged) {
+            for (int f = tid; f < nt; f += SIM_TILE) {
+                const int k = cell_of(f);
+                const int sj = mstart[k] + (f - mpre[k]);
+                qx[f] = (TC)x[(size_t)sj * STRIDE + 0] + msh[k][0];
+                qy[f] = (TC)x[(size_t)sj * STRIDE + 1] + msh[k][1];
+                qz[f] = (TC)x[(size_t)sj * STRIDE + 2] + msh[k][2];
+                qs[f] = sj;
+                q16[f] = (unsigned short)(mslot[k] + (f - mpre[k]));
+            }
+        }
+        __syncthreads();
+        const int h0 = mstart[13], nh = mcount[13];   // the cell itself
+        const TC rl2 = (TC)P.rl2;
+        const TC lo2 = rl2 * (TC)(1.0 - margin), hi2 = rl2 * (TC)(1.0 + margin);
+        constexpr int L = SIM_LIST_LANES;
+        for (int a0 = 0; a0 < nh; a0 += SIM_TILE / L) {
+            const int a = a0 + tid / L, l = tid % L;
+            const bool live = a < nh;
+            const int i = h0 + (live ? a : 0);
+            const TC xi = (TC)x[(size_t)i * STRIDE + 0];
+            const TC yi = (TC)x[(size_t)i * STRIDE + 1];
+            const TC zi = (TC)x[(size_t)i * STRIDE + 2];
+            const int f0 = (int)(((long long)l * nt) / L), f1 = (int)(((long long)(l + 1) * nt) / L);
+            // candidate f: index, image position, column slot; then k_list's test
+            auto test = [&](int f, int& sj, unsigned short& slot) {
+                TC px, py, pz;
+                if (staged) {
+                    sj = qs[f];
+                    px = qx[f];
+                    py = qy[f];
+                    pz = qz[f];
+                    slot = q16[f];
+                } else {
+                    const int k = cell_of(f);
+                    sj = mstart[k] + (f - mpre[k]);
+                    px = (TC)x[(size_t)sj * STRIDE + 0] + msh[k][0];
+                    py = (TC)x[(size_t)sj * STRIDE + 1] + msh[k][1];
+                    pz = (TC)x[(size_t)sj * STRIDE + 2] + msh[k][2];
+                    slot = (unsigned short)(mslot[k] + (f - mpre[k]));
+                }
+                if (sj == i) return false;
+                const TC ex = xi - px, ey = yi - py, ez = zi - pz;
+                const TC e2 = (ex * ex + ey * ey) + ez * ez;
+                if (e2 < lo2) return true;
+                if (e2 > hi2) return false;
+                const TC xj = (TC)x[(size_t)sj * STRIDE + 0];
+                const TC yj = (TC)x[(size_t)sj * STRIDE + 1];
+                const TC zj = (TC)x[(size_t)sj * STRIDE + 2];
+                TC dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                min_image_ref<TC>(dx, dy, dz, P);
+                return (dx * dx + dy * dy) + dz * dz < rl2;
+            };
+            // staged: one test pass into a bit mask of the lane's run (at most
+            // SIM_LIST_CAP / L = 128 candidates, 8 LDS reads in flight), the
+            // writes from the mask; otherwise a count pass and a write pass
+            constexpr int W = SIM_LIST_CAP / L / 32;
+            unsigned bits[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) bits[w] = 0u;
+            int mine = 0;
+            if (live && staged) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    for (int u0 = 0; u0 < 32; u0 += 8) {
+                        const int fb = f0 + 32 * w + u0;
+                        if (fb >= f1) break;
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            int sj;
+                            unsigned short slot;
+                            if (fb + u < f1 && test(fb + u, sj, slot)) bits[w] |= 1u << (u0 + u);
+                        }
+                    }
+                    mine += __builtin_popcount(bits[w]);
+                }
+            } else if (live) {
+                for (int f = f0; f < f1; ++f) {
+                    int sj;
+                    unsigned short slot;
+                    mine += test(f, sj, slot) ? 1 : 0;
+                }
+            }
+            int incl = mine;
+#pragma unroll
+            for (int o = 1; o < L; o <<= 1) {
+                const int v = __shfl_up(incl, o, L);
+                if (l >= o) incl += v;
+            }
+            const int cnt = __shfl(incl, (int)((threadIdx.x & 63) | (L - 1)), 64);
+            if (live) {
+                int k = incl - mine;
+                auto put = [&](int sj, unsigned short slot) {
+                    if (k < max_nbr) {
+                        nbr[(size_t)k * n_stride + i] = sj;
+                        if (nbr16 != nullptr) nbr16[(size_t)i * max_nbr + k] = slot;
+                    }
+                    ++k;
+                };
+                if (staged) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        unsigned m = bits[w];
+                        while (m != 0) {
+                            const int f = f0 + 32 * w + __builtin_ctz(m);
+                            m &= m - 1;
+                            put(qs[f], q16[f]);
+                        }
+                    }
+                } else {
+                    for (int f = f0; f < f1; ++f) {
+                        int sj;
+                        unsigned short slot;
+                        if (test(f, sj, slot)) put(sj, slot);
+                    }
+                }
+                if (l == 0) {
+                    nnbr[i] = cnt < max_nbr ? cnt : max_nbr;
+                    if (cnt > max_nbr) atomicOr(overflow, 1);
+                    if (pos_last != nullptr) {
+                        pos_last[(size_t)i * 4 + 0] = (double)x[(size_t)i * STRIDE + 0];
+                        pos_last[(size_t)i * 4 + 1] = (double)x[(size_t)i * STRIDE + 1];
+                        pos_last[(size_t)i * 4 + 2] = (double)x[(size_t)i * STRIDE + 2];
+                        pos_last[(size_t)i * 4 + 3] = 0.0;
+                    }
+                }
+            }
+        }
+        if (tid == 0 && stencil_max != nullptr) {
+            const int cmax = max(mcol[0], max(mcol[1], mcol[2]));
+            atomicMax(stencil_max, cmax);
+            atomicMax(stencil_max + 1, nt);
+            if (cmax > 16383) atomicOr(overflow, 8);
+        }
+        __syncthreads();   // LDS is reused by the next cell
+    }
+}
+
 template <typename TC, typename T, int STRIDE>
 int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, const T* x,
                double* pos_last, const int* flag, const int* cell_rep, int ncells, int* atom_cell,
@@ -479,16 +731,26 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                            ro->vel, ro->force, ro->order, reinterpret_cast<int4*>(ro->image),
                            atom_cell, cell_atoms, ro->slot_of, ro->vel_half_tmp, ro->vel_half);
     }
-    if (ro != nullptr && tiles != nullptr)   // positions wrapped and in cell order
+    if (ro != nullptr && tiles != nullptr) {   // positions wrapped and in cell order
+        // staged build for replicas with >= 3 cells per dimension (their cell
+        // prefix in atom_slot, free after the scatter); k_list for the rest
+        const bool staged = n_stride >= R + 1;
+        if (staged) {
+            hipLaunchKernelGGL(k_act_base, dim3(1), dim3(256), 0, st, R, reps, flag, atom_slot);
+            hipLaunchKernelGGL((k_list_staged<TC, T, STRIDE>), dim3(SIM_LIST_GRID), dim3(SIM_TILE),
+                               0, st, R, reps, atom_slot, x, cell_count, cell_start, nbr, nnbr,
+                               n_stride, max_nbr, overflow, pos_last, ro->nbr16, ro->stencil_max);
+        }
         hipLaunchKernelGGL((k_list<TC, T, STRIDE, true>), dim3(ntiles), dim3(SIM_TILE), 0, st,
                            reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,
                            nbr, nnbr, n_stride, max_nbr, overflow, pos_last, ro->nbr16,
-                           ro->stencil_max);
-    else
+                           ro->stencil_max, staged ? 1 : 0);
+    } else {
         hipLaunchKernelGGL((k_list<TC, T, STRIDE, false>), dim3(ntiles), dim3(SIM_TILE), 0, st,
                            reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,
                            nbr, nnbr, n_stride, max_nbr, overflow, pos_last,
-                           ro ? ro->nbr16 : nullptr, ro ? ro->stencil_max : nullptr);
+                           ro ? ro->nbr16 : nullptr, ro ? ro->stencil_max : nullptr, 0);
+    }
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

@@ -189,6 +189,19 @@ class ReplicaEngine:
         self.cell_rep_np = np.repeat(np.arange(R, dtype=np.int32), caps)
 
     @property
+    def family(self):
+        """Pair-kernel family: "cell", "rows" or "global"."""
+        return self._family
+
+    @family.setter
+    def family(self, fam):
+        # builds under the global list skip the column slots and the row
+        # prefix; a switch to an LDS family rebuilds them at the next build
+        if fam != "global" and getattr(self, "_family", fam) == "global":
+            self._stale_slots = True
+        self._family = fam
+
+    @property
     def tiled(self):
         return self.family != "global"
 
@@ -271,6 +284,7 @@ class ReplicaEngine:
             _TUNE_CACHE[key] = choice
             self.profile["autotune_ms"] = times
         self.family = choice
+        self._stale_slots = False   # no build since setup's, which wrote every slot
 
     def cell_args(self):
         """Leading arguments of the sim_*_cell entry points."""
@@ -442,10 +456,16 @@ class ReplicaEngine:
     def build_list(self, force=False):
         """Skin trigger (force=True: every replica) and the gated rebuild."""
         s = stream_ptr()
-        # the cell-row prefix is needed by the row kernels only (and at the
-        # forced build of setup, before the autotuner has chosen)
+        if getattr(self, "_stale_slots", False) and self.use_tiled:
+            force = True
+            self._stale_slots = False
+        # the cell-row prefix and the 16-bit column slots are read by the LDS
+        # kernel families only (and are built at the forced build of setup,
+        # before the autotuner has chosen)
+        lds = force or self.use_tiled
         self._ro.row_base = (self.row_base.data_ptr()
                              if force or (self.use_tiled and self.family == "rows") else None)
+        self._ro.nbr16 = self.nbr16.data_ptr() if lds else None
         call("sim_nl_trigger", self.R, ptr(self.reps), ptr(self.maxd2), ptr(self.flag),
              ptr(self.nbuild), int(bool(force)), s)
         call("sim_nl_build", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
@@ -552,6 +572,8 @@ class ReplicaEngine:
     def capture(self, virial=True):
         """Record one step into a HIP graph. Recording executes nothing, so the
         state is unchanged until the first replay."""
+        if getattr(self, "_stale_slots", False) and self.use_tiled:
+            self.build_list(force=True)     # refresh the LDS families' slots first
         g = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
