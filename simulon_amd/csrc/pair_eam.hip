@@ -70,19 +70,21 @@ __device__ __forceinline__ void tab_fill(float2* tl, const float4* __restrict__ 
     __syncthreads();
 }
 
-// Tile walk of one workgroup. Persistent (LT): the workgroups on XCD x
-// (blockIdx.x % 8 == x, grid a multiple of 8) take XCD x's contiguous run of
-// tiles round robin; otherwise one tile per workgroup (xcd_block).
+// Tile walk of one workgroup of TPW tile groups (256 threads each).
+// Persistent: the workgroups on XCD x (blockIdx.x % 8 == x, grid a multiple
+// of 8) take XCD x's contiguous run of tiles, TPW consecutive tiles per
+// workgroup and step; otherwise one tile per workgroup (xcd_block, TPW = 1).
+// Group g's tile is base + m + g while m + g < end.
 struct TileWalk {
     int base, m, end, step;
-    __device__ __forceinline__ TileWalk(bool persistent, int ntiles) {
+    __device__ __forceinline__ TileWalk(bool persistent, int ntiles, int tpw) {
         if (persistent) {
             const int x = blockIdx.x % SIM_NXCD;
             const int q = ntiles / SIM_NXCD, r = ntiles % SIM_NXCD;
             base = x * q + min(x, r);
             end = q + (x < r ? 1 : 0);
-            m = blockIdx.x / SIM_NXCD;
-            step = gridDim.x / SIM_NXCD;
+            m = (int)(blockIdx.x / SIM_NXCD) * tpw;
+            step = (int)(gridDim.x / SIM_NXCD) * tpw;
         } else {
             base = xcd_block(blockIdx.x, gridDim.x);
             m = 0;
@@ -91,6 +93,43 @@ struct TileWalk {
         }
     }
 };
+
+// block_sum of one 256-thread group g of a larger workgroup (same order, so
+// the same value as block_sum); lds holds [groups][4 * NV]; every thread of
+// the workgroup must call it.
+template <int NV>
+__device__ __forceinline__ void group_sum(double (&v)[NV], double* lds, int g, int lt) {
+    const int lane = lt & 63, wid = lt >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[(g * 4 + wid) * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (lt == 0) {
+        const double* p = lds + g * 4 * NV;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = (p[k] + p[NV + k]) + (p[2 * NV + k] + p[3 * NV + k]);
+    }
+}
+
+// Tile groups per workgroup of the persistent density pass: one table copy
+// in LDS serves them all, so more waves per CU share it.
+#ifndef SIM_EAM_TPW
+#define SIM_EAM_TPW 4
+#endif
+// List entries in flight per thread in the persistent kernels: at 16 waves
+// per CU fewer per thread are faster (measured: density 8, force 4).
+#ifndef SIM_EAM_LDS_BATCH
+#define SIM_EAM_LDS_BATCH 8
+#endif
+#ifndef SIM_EAM_FORCE_LDS_BATCH
+#define SIM_EAM_FORCE_LDS_BATCH 4
+#endif
+#ifndef SIM_EAM_FORCE_TPW
+#define SIM_EAM_FORCE_TPW 4
+#endif
 
 // ---- pass 1 ---------------------------------------------------------------
 // One batch of UB list entries starting at k (indices in j[]). MASKED: the
@@ -138,7 +177,7 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
                                              const int* __restrict__ nbr, int ns,
                                              const sim_eam_params_t& E, const float2* tl,
                                              float thr) {
-    constexpr int U = SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
+    constexpr int U = LT ? SIM_EAM_LDS_BATCH : SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
     const int ti = NT == 1 ? 0 : xi.w;
     int k = 0;
     int jn[U];
@@ -161,22 +200,24 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
     }
 }
 
-template <int NT, bool LT>
-__global__ void __launch_bounds__(SIM_TILE)
+template <int NT, bool LT, int TPW>
+__global__ void __launch_bounds__(SIM_TILE * TPW)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                   int ntiles, const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
                   float* __restrict__ rho_out, float* __restrict__ fp_out, int4* __restrict__ rec,
                   double* __restrict__ partial, float thr) {
     extern __shared__ float2 tl[];
-    __shared__ double lds[4 * 1];
+    __shared__ double lds[TPW * 4 * 1];
+    const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass1), E.n_r);
-    for (TileWalk w(LT, ntiles); w.m < w.end; w.m += w.step) {
-        const int tb = w.base + w.m;
+    for (TileWalk w(LT, ntiles, TPW); w.m < w.end; w.m += w.step) {
+        const bool have = w.m + g < w.end;
+        const int tb = w.base + w.m + (have ? g : 0);
         const SimTile t = tiles[tb];
         const sim_replica_t& P = reps[t.rep];
-        const int i = t.start + threadIdx.x;
-        const bool active = i < P.atom_base + P.natoms;
+        const int i = t.start + lt;
+        const bool active = have && i < P.atom_base + P.natoms;
         double etot = 0.0;
         if (active) {
             const QBox B = load_qbox(P);
@@ -203,9 +244,9 @@ __global__ void __launch_bounds__(SIM_TILE)
             etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
         }
         double v[1] = {etot};
-        block_sum<1>(v, lds);
-        if (threadIdx.x == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
-        __syncthreads();   // lds is reused by the next tile
+        group_sum<1>(v, lds, g, lt);
+        if (have && lt == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
+        __syncthreads();   // lds is reused by the next tiles
     }
 }
 
@@ -291,7 +332,7 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
                                            const int4* __restrict__ typ, const int* __restrict__ nbr,
                                            int ns, const sim_eam_params_t& E, const float2* tl,
                                            float thr) {
-    constexpr int U = SIM_EAM_FORCE_BATCH, U2 = SIM_NB_BATCH;
+    constexpr int U = LT ? SIM_EAM_FORCE_LDS_BATCH : SIM_EAM_FORCE_BATCH, U2 = SIM_NB_BATCH;
     const float fpi = __int_as_float(xi.w);
     int k = 0;
     int jn[U];
@@ -315,22 +356,24 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
     }
 }
 
-template <int NT, bool LT, bool VIRIAL>
-__global__ void __launch_bounds__(SIM_TILE)
+template <int NT, bool LT, bool VIRIAL, int TPW>
+__global__ void __launch_bounds__(SIM_TILE * TPW)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                 int ntiles, const int4* __restrict__ typ, const int4* __restrict__ rec,
                 const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
                 sim_eam_params_t E, float* __restrict__ force, double* __restrict__ partial,
                 float thr) {
     extern __shared__ float2 tl[];
-    __shared__ double lds[4 * 7];
+    __shared__ double lds[TPW * 4 * 7];
+    const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass2), E.n_r);
-    for (TileWalk w(LT, ntiles); w.m < w.end; w.m += w.step) {
-        const int tb = w.base + w.m;
+    for (TileWalk w(LT, ntiles, TPW); w.m < w.end; w.m += w.step) {
+        const bool have = w.m + g < w.end;
+        const int tb = w.base + w.m + (have ? g : 0);
         const SimTile t = tiles[tb];
         const sim_replica_t& P = reps[t.rep];
-        const int i = t.start + threadIdx.x;
-        const bool active = i < P.atom_base + P.natoms;
+        const int i = t.start + lt;
+        const bool active = have && i < P.atom_base + P.natoms;
         ForceAcc a;
         if (active) {
             const QBox B = load_qbox(P);
@@ -349,13 +392,13 @@ __global__ void __launch_bounds__(SIM_TILE)
         }
         double v[7] = {0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
                        0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
-        block_sum<7>(v, lds);
-        if (threadIdx.x == 0) {
+        group_sum<7>(v, lds, g, lt);
+        if (have && lt == 0) {
             double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
             for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
         }
-        __syncthreads();   // lds is reused by the next tile
+        __syncthreads();   // lds is reused by the next tiles
     }
 }
 
@@ -380,13 +423,14 @@ bool use_lds_tab(const sim_eam_params_t& E, int ntiles) {
            ntiles >= SIM_EAM_LDS_MIN_TILES;
 }
 
-int persistent_grid(int ntiles) {
+// units: work items (tiles, or tile groups); per_cu: workgroups per CU.
+int persistent_grid(int units, int per_cu = 2) {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         ncu <= 0)
         ncu = 256;
-    const int g = min(ntiles, 2 * ncu);
+    const int g = min(units, per_cu * ncu);
     return (g + SIM_NXCD - 1) / SIM_NXCD * SIM_NXCD;
 }
 
@@ -416,17 +460,19 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     auto* W = reinterpret_cast<int4*>(rec);
     hipStream_t st = as_stream(stream);
     const size_t tb = lds_tab_bytes(eam);
-    static const bool lds_ok = allow_lds(k_eam_density<1, true>);
+    static const bool lds_ok = allow_lds(k_eam_density<1, true, SIM_EAM_TPW>);
     if (use_lds_tab(eam, ntiles) && lds_ok)
-        hipLaunchKernelGGL((k_eam_density<1, true>), dim3(persistent_grid(ntiles)),
-                           dim3(SIM_TILE), tb, st, P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho,
-                           fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<1, true, SIM_EAM_TPW>),
+                           dim3(persistent_grid((ntiles + SIM_EAM_TPW - 1) / SIM_EAM_TPW,
+                                                2 / SIM_EAM_TPW > 0 ? 2 / SIM_EAM_TPW : 1)),
+                           dim3(SIM_TILE * SIM_EAM_TPW), tb, st, P, T, ntiles, X, nbr, nnbr,
+                           n_stride, eam, rho, fp, W, partial, thr);
     else if (eam.ntypes == 1)
-        hipLaunchKernelGGL((k_eam_density<1, false>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T,
-                           ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<1, false, 1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P,
+                           T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
     else
-        hipLaunchKernelGGL((k_eam_density<0, false>), dim3(ntiles), dim3(SIM_TILE), 0, st, P, T,
-                           ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<0, false, 1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P,
+                           T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
@@ -450,16 +496,20 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     const size_t tb = lds_tab_bytes(eam);
     auto go = [&](auto v) {
         constexpr bool V = decltype(v)::value;
-        static const bool lds_ok = allow_lds(k_eam_force<1, true, V>);
+        constexpr int TPW = SIM_EAM_FORCE_TPW;
+        static const bool lds_ok = allow_lds(k_eam_force<1, true, V, TPW>);
         if (use_lds_tab(eam, ntiles) && lds_ok)
-            hipLaunchKernelGGL((k_eam_force<1, true, V>), dim3(persistent_grid(ntiles)), b, tb, st,
-                               P, T, ntiles, X, Q, nbr, nnbr, n_stride, eam, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW>),
+                               dim3(persistent_grid((ntiles + TPW - 1) / TPW,
+                                                    2 / TPW > 0 ? 2 / TPW : 1)),
+                               dim3(SIM_TILE * TPW), tb, st, P, T, ntiles, X, Q, nbr, nnbr,
+                               n_stride, eam, force, partial, thr);
         else if (eam.ntypes == 1)
-            hipLaunchKernelGGL((k_eam_force<1, false, V>), g, b, 0, st, P, T, ntiles, X, Q, nbr,
-                               nnbr, n_stride, eam, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<1, false, V, 1>), g, b, 0, st, P, T, ntiles, X, Q,
+                               nbr, nnbr, n_stride, eam, force, partial, thr);
         else
-            hipLaunchKernelGGL((k_eam_force<0, false, V>), g, b, 0, st, P, T, ntiles, X, Q, nbr,
-                               nnbr, n_stride, eam, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<0, false, V, 1>), g, b, 0, st, P, T, ntiles, X, Q,
+                               nbr, nnbr, n_stride, eam, force, partial, thr);
     };
     if (virial) go(std::true_type());
     else go(std::false_type());
