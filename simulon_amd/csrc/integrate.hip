@@ -48,8 +48,12 @@ __global__ void __launch_bounds__(SIM_TILE)
     __syncthreads();
     if (threadIdx.x == 0) {
         const double m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
-        // non-negative doubles order like their bit patterns
-        atomicMax(&maxd2[t.rep], (unsigned long long)__double_as_longlong(m));
+        // non-negative doubles order like their bit patterns; the plain read
+        // first skips the atomic (contended when one replica has thousands of
+        // tiles) whenever it could not raise the maximum (a stale read only
+        // costs an extra atomic)
+        const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+        if (b > maxd2[t.rep]) atomicMax(&maxd2[t.rep], b);
     }
 }
 

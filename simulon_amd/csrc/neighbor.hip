@@ -432,10 +432,11 @@ __global__ void __launch_bounds__(SIM_TILE)
     }
     nnbr[i] = cnt < max_nbr ? cnt : max_nbr;
     if (cnt > max_nbr) atomicOr(overflow, 1);
-    if (stencil_max != nullptr) {
+    if (stencil_max != nullptr) {   // read first: rarely raised, the atomics contend
         const int cmax = max(coff[0], max(coff[1], coff[2]));
-        atomicMax(stencil_max, cmax);                               // largest column
-        atomicMax(stencil_max + 1, coff[0] + coff[1] + coff[2]);    // largest stencil
+        const int ctot = coff[0] + coff[1] + coff[2];
+        if (cmax > stencil_max[0]) atomicMax(stencil_max, cmax);       // largest column
+        if (ctot > stencil_max[1]) atomicMax(stencil_max + 1, ctot);   // largest stencil
         if (cmax > 16383) atomicOr(overflow, 8);   // offsets no longer fit 14 bits
     }
     if (pos_last != nullptr) {
@@ -684,8 +685,8 @@ __global__ void __launch_bounds__(SIM_TILE)
         }
         if (tid == 0 && stencil_max != nullptr) {
             const int cmax = max(mcol[0], max(mcol[1], mcol[2]));
-            atomicMax(stencil_max, cmax);
-            atomicMax(stencil_max + 1, nt);
+            if (cmax > stencil_max[0]) atomicMax(stencil_max, cmax);   // read first: rarely raised
+            if (nt > stencil_max[1]) atomicMax(stencil_max + 1, nt);
             if (cmax > 16383) atomicOr(overflow, 8);
         }
         __syncthreads();   // LDS is reused by the next cell
