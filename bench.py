@@ -241,6 +241,37 @@ def cpu_baseline_eam(setfl, natoms_side=10, seconds=15.0):
                       f"single-threaded)"}
 
 
+def cpu_baseline_lj(pos, H, dt, seconds=15.0, mode="raw"):
+    """The oracle's float32 LJ + BAOB step (edge list built once, as between
+    list rebuilds) on `pos`, timed on the host."""
+    from oracle import integrate as oint
+    from oracle import lj as olj
+    from oracle import neighbor as onb
+    x = np.asarray(pos, np.float32)
+    n = len(x)
+    ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(8.5, 1.0))
+    types = np.zeros(n, int)
+    m = oint.internal_mass(np.full(n, lat.AR_MASS))
+    rng = np.random.default_rng(0)
+    v = oint.maxwell_boltzmann(np.full(n, lat.AR_MASS), 94.4, rng)
+    ff = lambda x_: olj.lj_forward(x_, ei, types, [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, H,
+                                   mode=mode)["forces"]
+    f = ff(x)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        vh, x = oint.first_half(x, v, f, m, dt)
+        f = ff(x)
+        v, _, _ = oint.second_half(vh, f, m, dt, gamma=10.0, T=94.4,
+                                   noise=rng.standard_normal((n, 3)).astype(np.float32))
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": round(steps * n / el, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle float32 LJ + BAOB step, {n} Ar atoms, {steps} steps in {el:.1f} s, "
+                      f"list built once (host {os.cpu_count()} cores visible, numpy "
+                      f"single-threaded)"}
+
+
 def bench_replicas(args):
     tmp = tempfile.mkdtemp()
     setfl = lat.write_w_synthetic_setfl(os.path.join(tmp, "W_synthetic.eam.alloy"))
@@ -346,6 +377,15 @@ def bench_single(args):
             f"{drift:.3e} eV/atom/ns")
     roof = lj_kernel_roofline(eng, w) if isinstance(eng.force_field, LJForceField) else \
         eam_kernel_roofline(eng, w)
+    cpu = None
+    if not args.no_cpu:
+        if w == "lj_argon_nvt":
+            cpu = cpu_baseline_lj(pos, H, dt_ps, seconds=args.cpu_seconds)
+        elif w == "lj_argon_npt_tri":   # a 2,048-atom block of the same lattice and shear
+            cpu = cpu_baseline_lj(*lat.lj_argon_npt_tri(ncell=8, seed=0), dt_ps,
+                                  seconds=args.cpu_seconds)
+        else:
+            cpu = cpu_baseline_eam(setfl, seconds=args.cpu_seconds)
     return {"metric": f"atom-steps/s ({w})", "value": round(float(rate), 1), "unit": "atom-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(t * 1e3 / args.steps, 4), "higher_is_better": True,
@@ -357,7 +397,7 @@ def bench_single(args):
                 eng.N, t * 1e3 / args.steps,
                 (12 + 4 * roof["nbar_full"] + 12 + 88) if isinstance(eng.force_field, LJForceField)
                 else 132 + 8 * roof["nbar_full"]),
-            "nve_drift_eV_per_atom_per_ns": drift}
+            "nve_drift_eV_per_atom_per_ns": drift, "cpu_baseline": cpu}
 
 
 def mlp_kernel_roofline(eng):
