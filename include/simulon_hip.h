@@ -298,10 +298,13 @@ int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
 /* Second half (O + B): v = vh damp + xi rff (Langevin replicas), then
  * v += 1/2 F/m dt; KE and kinetic-tensor columns of partial.
  * noise: optional float[N][3] of N(0,1) draws (NULL = Philox4x32-10 keyed by
- * seed, counter = (global atom index, step)). */
+ * seed, counter = (atom id, step)). key: optional int64[N] caller index of the
+ * atom in each slot; the atom id is then key - atom_base of its replica (the
+ * same draws for an atom whatever its storage slot, and in a packed engine as
+ * in a single-replica one); NULL: the slot index. */
 int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
                          sim_types_t types, const double* pos4, const float* vel_half,
-                         const float* force, float* vel, const float* noise,
+                         const float* force, float* vel, const float* noise, const long long* key,
                          unsigned long long seed, unsigned long long step,
                          const unsigned long long* step_dev, double* partial, void* stream);
 

@@ -105,7 +105,7 @@ SIGNATURES = {
     "sim_eam_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _I,
                            _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,
+    "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,
                                   _vp, _vp]),
     "sim_step_advance": (_I, [_vp, _vp]),
     "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp]),

@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(SIM_TILE)
     k_second(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
              sim_types_t ty, const double4* __restrict__ pos, const float* __restrict__ vh,
              const float* __restrict__ force, float* __restrict__ vel,
-             const float* __restrict__ noise, unsigned long long seed, unsigned long long step0,
+             const float* __restrict__ noise, const long long* __restrict__ key,
+             unsigned long long seed, unsigned long long step0,
              const unsigned long long* __restrict__ step_dev, double* __restrict__ partial) {
     const unsigned long long step = step0 + (step_dev ? step_dev[0] : 0ull);
     __shared__ double lds[4 * 7];
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(SIM_TILE)
                 g1 = noise[o + 1];
                 g2 = noise[o + 2];
             } else {
-                gauss3(seed, step, (uint32_t)i, g0, g1, g2);
+                const uint32_t id = key ? (uint32_t)(key[i] - P.atom_base) : (uint32_t)i;
+                gauss3(seed, step, id, g0, g1, g2);
             }
             const float s = P.rff[ty_i];
             v0 = v0 * P.damp + g0 * s;
@@ -327,6 +329,7 @@ extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntile
 extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
                                     sim_types_t types, const double* pos4, const float* vel_half,
                                     const float* force, float* vel, const float* noise,
+                                    const long long* key,
                                     unsigned long long seed, unsigned long long step,
                                     const unsigned long long* step_dev, double* partial,
                                     void* stream) {
@@ -336,7 +339,7 @@ extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntil
     hipLaunchKernelGGL(k_second, dim3(ntiles), dim3(SIM_TILE), 0, as_stream(stream),
                        static_cast<const sim_replica_t*>(reps),
                        reinterpret_cast<const SimTile*>(tiles), types,
-                       reinterpret_cast<const double4*>(pos4), vel_half, force, vel, noise, seed,
+                       reinterpret_cast<const double4*>(pos4), vel_half, force, vel, noise, key, seed,
                        step, step_dev, partial);
     SIM_CHECK_LAUNCH();
     return SIM_OK;

@@ -494,7 +494,7 @@ class ReplicaEngine:
         s = stream_ptr()
         call("sim_integrate_second", ptr(self.reps), ptr(self.tiles), self.ntiles,
              self.types_struct, ptr(self.pos4), ptr(self.vel_half), ptr(self.force), ptr(self.vel),
-             ptr(noise), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
+             ptr(noise), ptr(self.order), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
         call("sim_step_advance", ptr(self.step_ctr), s)
 
     def reduce(self, col0=0, col1=15, ff=None):
