@@ -1,0 +1,69 @@
+"""Energy minimisers (simulon_amd.energy_minimizer; the reference's
+core/energy_minimizer.py): a jittered LJ argon fcc crystal (a local energy
+minimum in its fixed box) relaxes back to the lattice, whose energy the
+float64 oracle gives, and the energy returned is the oracle's at the returned
+positions."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle_lj
+from simulon_amd import lattice as lat
+
+pytestmark = pytest.mark.gpu
+
+A_FCC = 5.26      # near the LJ argon fcc spacing for rc = 8.5 A
+NCELL = 5         # 500 atoms, box 26.3 A
+
+
+def _crystal(amp=0.0, seed=None):
+    H = np.eye(3) * A_FCC * NCELL
+    pos = lat.lattice(lat.FCC_BASIS, NCELL, H)
+    return (lat.jitter(pos, amp, seed) if amp else pos), H
+
+
+def _energy(pos, H):
+    return oracle_lj(pos, H, np.zeros(len(pos), int), [[0.0104]], [[3.405]], 8.5, 1.0)["energy"]
+
+
+def _model(pos, H):
+    from simulon_amd.force import LennardJonesForce
+    from simulon_amd.integrator import VerletIntegrator
+    from simulon_amd.md_model import BaseModel, SumBackboneInterface
+    from simulon_amd.reader import AtomFileReader
+    mol = AtomFileReader(atom_types=["Ar"] * len(pos), positions=pos, box_length=float(H[0, 0]),
+                         cutoff=8.5, device="cuda",
+                         parameter={"[0 0]": {"epsilon": 0.0104, "sigma": 3.405}},
+                         skin_thickness=1.0)
+    ff = LennardJonesForce(mol)
+    return mol, BaseModel(SumBackboneInterface([ff], mol), VerletIntegrator(mol, dt=0.002), mol)
+
+
+@pytest.mark.parametrize("method", ["steepest", "lbfgs"])
+def test_minimizer_relaxes_to_lattice(method):
+    from simulon_amd.energy_minimizer import (minimize_energy_bfgs_scipy,
+                                              minimize_energy_steepest_descent)
+    print(f"[case] minimize {method}", flush=True)
+    pos0, H = _crystal()
+    e_lat = _energy(pos0, H)
+    pos, _ = _crystal(amp=0.05, seed=5)
+    e_start = _energy(pos, H)
+    mol, model = _model(pos, H)
+    if method == "steepest":
+        e = minimize_energy_steepest_descent(model, max_steps=300, step_size=0.002,
+                                             force_threshold=1e-3)
+        hist = np.array(model.minimize_history)
+        assert np.all(np.diff(hist[:, 0]) < 0)          # every accepted step lowers E
+        tol = 0.05
+    else:
+        e = minimize_energy_bfgs_scipy(model, max_iter=100)
+        tol = 1e-3
+    eng = model.engine
+    x = eng.to_caller(eng.positions(torch.float64, unwrap=True)).cpu().numpy()
+    e_ref = _energy(x, H)
+    fmax = float(eng.force.norm(dim=1).max())
+    print(f"{method}: E start {e_start:.6f} -> {e:.6f} (oracle at the result {e_ref:.6f}), "
+          f"lattice {e_lat:.6f}; max |F| {fmax:.2e}", flush=True)
+    assert e_lat - 1e-6 * abs(e_lat) < e < e_start          # float32 energy: 1e-6 rel
+    assert abs(e - e_ref) <= 1e-6 * abs(e_ref)            # energy of the returned positions
+    assert e - e_lat <= tol * (e_start - e_lat)            # back to the lattice
