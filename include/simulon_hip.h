@@ -41,6 +41,9 @@
  *        force fields (e.g. core/force/lennard_jones_force.py:92-94),
  *        core/md_model.py:83-86 and integrator.py:118-119
  *   sim_berendsen                  core/barostat.py:36-68
+ *   sim_aniso_barostat             AnisotropicNPTBarostat.step, core/barostat.py:131-208
+ *   sim_box_deform                 UniaxialTensileLoader.step,
+ *        core/mechanics/loading.py:39-78
  *   sim_mlp_*                      the learned-potential backbone slot of
  *        machine_learning_potentials/machine_learning_force.py:85-142 (the
  *        network itself is this library's own, see the MLP section below)
@@ -78,7 +81,8 @@ typedef struct sim_replica {
     double trig2;         /* squared rebuild displacement (skin/2)^2 */
     double rl;            /* list radius */
     double skin;
-    double mu_acc;        /* isotropic box scaling accumulated since the last build */
+    double mu_acc;        /* lower bound on how much the box changes since the last
+                             build shrink any separation (smallest singular value) */
     double dt;            /* time step, ps */
     double mu;            /* last Berendsen scaling factor */
     double press;         /* last instantaneous pressure, bar */
@@ -95,8 +99,24 @@ typedef struct sim_replica {
     int ncell[3];
     int ortho;
     int langevin;         /* 1: Langevin O step in the second half */
-    int barostat;         /* 1: Berendsen after the second half */
+    int barostat;         /* 1: Berendsen, 2: anisotropic, after the second half */
     int pad;
+    /* anisotropic barostat (core/barostat.py:78-208) and uniaxial loading
+     * (core/mechanics/loading.py:39-78) */
+    double eta_dot[3];    /* log strain rates of the lattice rows, 1/ps */
+    double p_target3[3];  /* target pressure per lattice axis, bar */
+    double p_axis[3];     /* last pressure projected on the lattice axes, bar */
+    double gamma_p;       /* strain-rate friction, 1/ps */
+    double max_eta_dot;
+    double p_tol;         /* deadband around the target, bar */
+    double temp_k;        /* temperature of the stochastic term, K */
+    double deform[3];     /* loading: row scale factors applied before every step */
+    double amap[9];       /* affine map x' = x amap of the last row scaling */
+    double hb_inv[9];     /* hinv at the last list build (rebuild trigger bound) */
+    float vscale[3];      /* per-axis velocity factors of the last row scaling */
+    int axes;             /* anisotropic: bit a set = lattice row a controlled */
+    int stochastic;
+    int loading;          /* 1: apply deform at the start of every step */
 } sim_replica_t;
 
 typedef struct sim_types {
@@ -323,6 +343,22 @@ int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int 
  * pos_last scaled by mu. */
 int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double* thermo,
                   double* pos4, float* vel, double* pos_last, void* stream);
+
+/* Anisotropic barostat (barostat.py:131-208) for replicas with barostat = 2:
+ * the pressure tensor (K + W)/V projected on the unit lattice rows, the
+ * deadband/friction/clamp update of eta_dot (with a Philox N(0,1) term keyed
+ * by (seed, step_dev, replica) when stochastic), lattice rows scaled by
+ * exp(dt eta_dot), x and pos_last mapped affinely with the box, v scaled by
+ * exp(-dt eta_dot) per Cartesian axis; p_axis holds the projected pressure. */
+int sim_aniso_barostat(int R, void* reps, const int* tiles, int ntiles, const double* thermo,
+                       double* pos4, float* vel, double* pos_last, unsigned long long seed,
+                       const unsigned long long* step_dev, void* stream);
+
+/* Uniaxial loading (loading.py:39-78) for replicas with loading = 1: lattice
+ * rows scaled by deform[3], x and pos_last mapped affinely with the box
+ * (fractional coordinates unchanged); velocities are not scaled. */
+int sim_box_deform(int R, void* reps, const int* tiles, int ntiles, double* pos4,
+                   double* pos_last, void* stream);
 
 /* Copies positions into a float32 [N,3] (dtype 0) or float64 [N,3] (dtype 1)
  * array, and back (sim_pos_import). */

@@ -16,6 +16,9 @@ PyTorch CPU path for one MD step:
                  io_utils/reader.py:319-339,
                  core/md_model.py:83-166,            BAOB Langevin velocity Verlet,
                  core/barostat.py:36-68              Berendsen scaling factor
+  mechanics.py   core/barostat.py:78-208,
+                 core/mechanics/loading.py:39-78     anisotropic barostat,
+                                                     uniaxial tensile loading
   mlp.py         radial-descriptor MLP potential (no reference counterpart:
                  the reference's learned potential is an external CHGNet
                  wrapper; this restates the library's own network in FP64)

@@ -36,6 +36,12 @@ class Replica(ctypes.Structure):
         ("atom_base", ctypes.c_int), ("natoms", ctypes.c_int), ("cell_base", ctypes.c_int),
         ("cell_cap", ctypes.c_int), ("ncell", ctypes.c_int * 3), ("ortho", ctypes.c_int),
         ("langevin", ctypes.c_int), ("barostat", ctypes.c_int), ("pad", ctypes.c_int),
+        ("eta_dot", ctypes.c_double * 3), ("p_target3", ctypes.c_double * 3),
+        ("p_axis", ctypes.c_double * 3), ("gamma_p", ctypes.c_double),
+        ("max_eta_dot", ctypes.c_double), ("p_tol", ctypes.c_double), ("temp_k", ctypes.c_double),
+        ("deform", ctypes.c_double * 3), ("amap", ctypes.c_double * 9),
+        ("hb_inv", ctypes.c_double * 9), ("vscale", ctypes.c_float * 3), ("axes", ctypes.c_int),
+        ("stochastic", ctypes.c_int), ("loading", ctypes.c_int),
     ]
 
 
@@ -110,6 +116,8 @@ SIGNATURES = {
     "sim_step_advance": (_I, [_vp, _vp]),
     "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp]),
     "sim_berendsen": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp]),
+    "sim_aniso_barostat": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
+    "sim_box_deform": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp]),
     "sim_pos_export": (_I, [_vp, _I, _vp, _I, _vp]),
     "sim_pos_import": (_I, [_vp, _I, _vp, _I, _vp]),
     "sim_mlp_energy": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, MLPNet, _vp, _vp, _vp, _vp, _vp]),

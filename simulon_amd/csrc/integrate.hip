@@ -205,7 +205,7 @@ __global__ void k_baro_box(int R, sim_replica_t* reps, const double* thermo) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     sim_replica_t& P = reps[r];
-    if (!P.barostat) {
+    if (P.barostat != 1) {
         P.mu = 1.0;
         return;
     }
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(SIM_TILE)
                  double4* __restrict__ pos, float* __restrict__ vel, double4* __restrict__ pos_last) {
     const SimTile t = tile_of(tiles);
     const sim_replica_t& P = reps[t.rep];
-    if (!P.barostat) return;
+    if (P.barostat != 1) return;
     const int i = t.start + threadIdx.x;
     if (i >= P.atom_base + P.natoms) return;
     const double mu = P.mu;
@@ -252,6 +252,159 @@ __global__ void __launch_bounds__(SIM_TILE)
     vel[o] *= muf;
     vel[o + 1] *= muf;
     vel[o + 2] *= muf;
+}
+
+// ---- lattice-row scaling: anisotropic barostat and uniaxial loading ----------
+//
+// Both scale the lattice rows, H' = diag(s) H, and carry the atoms with the
+// box: x' = x H^-1 H' (fractional coordinates unchanged), pos_last too so that
+// the rebuild trigger keeps measuring displacements in the co-moving frame.
+
+// Smallest eigenvalue of the symmetric 3x3 G (xx yy zz xy xz yz) by the
+// trigonometric closed form (O. K. Smith, CACM 4(4):168, 1961).
+__device__ double sym3_min_eig(double a, double b, double c, double d, double e, double f) {
+    const double p1 = d * d + e * e + f * f;
+    if (p1 == 0.0) return fmin(a, fmin(b, c));
+    const double q = (a + b + c) / 3.0;
+    const double p2 = (a - q) * (a - q) + (b - q) * (b - q) + (c - q) * (c - q) + 2.0 * p1;
+    const double p = sqrt(p2 / 6.0);
+    const double ba = (a - q) / p, bb = (b - q) / p, bc = (c - q) / p;
+    const double bd = d / p, be = e / p, bf = f / p;
+    const double det = ba * (bb * bc - bf * bf) - bd * (bd * bc - bf * be) + be * (bd * bf - bb * be);
+    const double r = fmin(1.0, fmax(-1.0, 0.5 * det));
+    const double phi = acos(r) / 3.0;
+    return q + 2.0 * p * cos(phi + 2.0943951023931957);     // + 2 pi / 3
+}
+
+// mu_acc = smallest singular value of F = Hb^-1 H (rows map as d' = d F), the
+// factor by which the box change since the last build can shrink a
+// separation, lowered by 1e-12 to stay a bound under rounding.
+__device__ void update_mu_acc(sim_replica_t& P) {
+    double F[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            F[3 * i + j] = P.hb_inv[3 * i] * P.h[j] + P.hb_inv[3 * i + 1] * P.h[3 + j] +
+                           P.hb_inv[3 * i + 2] * P.h[6 + j];
+    auto dot = [&](int i, int k) {
+        return F[3 * i] * F[3 * k] + F[3 * i + 1] * F[3 * k + 1] + F[3 * i + 2] * F[3 * k + 2];
+    };
+    const double lam = sym3_min_eig(dot(0, 0), dot(1, 1), dot(2, 2), dot(0, 1), dot(0, 2), dot(1, 2));
+    P.mu_acc = sqrt(fmax(lam, 0.0)) - 1e-12;
+}
+
+// H' = diag(s) H; amap = H^-1 H'; hinv' = H^-1 diag(1/s); volume scales by
+// s0 s1 s2.
+__device__ void scale_rows(sim_replica_t& P, const double s[3]) {
+    double hn[9];
+    for (int a = 0; a < 3; ++a)
+        for (int j = 0; j < 3; ++j) hn[3 * a + j] = s[a] * P.h[3 * a + j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            P.amap[3 * i + j] = P.hinv[3 * i] * hn[j] + P.hinv[3 * i + 1] * hn[3 + j] +
+                                P.hinv[3 * i + 2] * hn[6 + j];
+    for (int k = 0; k < 9; ++k) P.h[k] = hn[k];
+    for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 3; ++a) P.hinv[3 * i + a] /= s[a];
+    P.volume *= s[0] * s[1] * s[2];
+    update_mu_acc(P);
+}
+
+// core/barostat.py:131-208 for replicas with barostat = 2.
+__global__ void k_aniso_box(int R, sim_replica_t* reps, const double* thermo,
+                            unsigned long long seed, const unsigned long long* step_dev) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    sim_replica_t& P = reps[r];
+    if (P.barostat != 2) return;
+    const double* th = thermo + (size_t)r * SIM_NPART;
+    const double bar = 1602176.6208;
+    // (K + W) / V, symmetric, in eV/A^3
+    double S[9];
+    const int ia[6] = {0, 4, 8, 1, 2, 5}, ib[6] = {0, 4, 8, 3, 6, 7};
+    for (int k = 0; k < 6; ++k) {
+        const double v = (th[SIM_T_K + k] + th[SIM_T_W + k]) / P.volume;
+        S[ia[k]] = v;
+        S[ib[k]] = v;
+    }
+    float gn[3] = {0.f, 0.f, 0.f};
+    if (P.stochastic) {
+        const unsigned long long step = step_dev ? step_dev[0] : 0ull;
+        gauss3(seed, step, (uint32_t)r, gn[0], gn[1], gn[2], 0xba20u);
+    }
+    const double tau2 = fmax(P.tau_p * P.tau_p, 1e-12);
+    bool within[3];
+    bool any_within = false;
+    double accel[3];
+    for (int a = 0; a < 3; ++a) {
+        const double* u = P.h + 3 * a;
+        const double n = fmax(sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), 1e-12);
+        const double e0 = u[0] / n, e1 = u[1] / n, e2 = u[2] / n;
+        const double pa = e0 * (S[0] * e0 + S[1] * e1 + S[2] * e2) +
+                          e1 * (S[3] * e0 + S[4] * e1 + S[5] * e2) +
+                          e2 * (S[6] * e0 + S[7] * e1 + S[8] * e2);
+        const double pbar = pa * bar;
+        P.p_axis[a] = pbar;
+        const double dp = pbar - P.p_target3[a];
+        within[a] = fabs(dp) <= P.p_tol;
+        any_within |= within[a];
+        accel[a] = P.kappa * (within[a] ? 0.0 : dp) / tau2;
+        if (P.gamma_p > 0.0) accel[a] -= P.gamma_p * P.eta_dot[a];
+        if (P.stochastic)
+            accel[a] += (double)gn[a] * (P.kappa * 8.617333262e-5 * fmax(P.temp_k, 1.0)) /
+                        fmax(P.tau_p, 1e-12);
+    }
+    const double hold = fmax(P.gamma_p, 5.0);
+    double s[3];
+    for (int a = 0; a < 3; ++a) {
+        if (any_within && within[a]) accel[a] = -hold * P.eta_dot[a];
+        const bool ctl = (P.axes >> a) & 1;
+        double e = P.eta_dot[a] + P.dt * accel[a];
+        e = fmin(fmax(e, -P.max_eta_dot), P.max_eta_dot);
+        if (!ctl) e = 0.0;
+        if (within[a] && fabs(e) < 1e-7) e = 0.0;
+        P.eta_dot[a] = e;
+        s[a] = ctl ? exp(P.dt * e) : 1.0;
+        P.vscale[a] = ctl ? (float)exp(-P.dt * e) : 1.0f;
+    }
+    scale_rows(P, s);
+    P.press = (P.p_axis[0] + P.p_axis[1] + P.p_axis[2]) / 3.0;
+}
+
+// core/mechanics/loading.py:47-78 for replicas with loading = 1.
+__global__ void k_deform_box(int R, sim_replica_t* reps) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    sim_replica_t& P = reps[r];
+    if (!P.loading) return;
+    const double s[3] = {P.deform[0], P.deform[1], P.deform[2]};
+    scale_rows(P, s);
+}
+
+// x' = x amap for pos and pos_last; with VEL, v_a *= vscale[a]. MODE 2 serves
+// the replicas with barostat = 2, MODE 3 those with loading = 1.
+template <int MODE, bool VEL>
+__global__ void __launch_bounds__(SIM_TILE)
+    k_affine_map(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+                 double4* __restrict__ pos, float* __restrict__ vel,
+                 double4* __restrict__ pos_last) {
+    const SimTile t = tile_of(tiles);
+    const sim_replica_t& P = reps[t.rep];
+    if (MODE == 2 ? P.barostat != 2 : !P.loading) return;
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const double* M = P.amap;
+    double4 x = pos[i];
+    pos[i] = double4{x.x * M[0] + x.y * M[3] + x.z * M[6], x.x * M[1] + x.y * M[4] + x.z * M[7],
+                     x.x * M[2] + x.y * M[5] + x.z * M[8], x.w};
+    double4 l = pos_last[i];
+    pos_last[i] = double4{l.x * M[0] + l.y * M[3] + l.z * M[6], l.x * M[1] + l.y * M[4] + l.z * M[7],
+                          l.x * M[2] + l.y * M[5] + l.z * M[8], l.w};
+    if (VEL) {
+        const size_t o = (size_t)i * 3;
+        vel[o] *= P.vscale[0];
+        vel[o + 1] *= P.vscale[1];
+        vel[o + 2] *= P.vscale[2];
+    }
 }
 
 template <typename T>
@@ -392,6 +545,37 @@ extern "C" int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, co
                        static_cast<const sim_replica_t*>(reps),
                        reinterpret_cast<const SimTile*>(tiles), reinterpret_cast<double4*>(pos4), vel,
                        reinterpret_cast<double4*>(pos_last));
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_aniso_barostat(int R, void* reps, const int* tiles, int ntiles,
+                                  const double* thermo, double* pos4, float* vel,
+                                  double* pos_last, unsigned long long seed,
+                                  const unsigned long long* step_dev, void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !thermo || !pos4 || !vel || !pos_last)
+        return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_aniso_box, dim3((R + 63) / 64), dim3(64), 0, st, R,
+                       static_cast<sim_replica_t*>(reps), thermo, seed, step_dev);
+    hipLaunchKernelGGL((k_affine_map<2, true>), dim3(ntiles), dim3(SIM_TILE), 0, st,
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles), reinterpret_cast<double4*>(pos4), vel,
+                       reinterpret_cast<double4*>(pos_last));
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
+}
+
+extern "C" int sim_box_deform(int R, void* reps, const int* tiles, int ntiles, double* pos4,
+                              double* pos_last, void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || !pos4 || !pos_last) return SIM_E_ARG;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_deform_box, dim3((R + 63) / 64), dim3(64), 0, st, R,
+                       static_cast<sim_replica_t*>(reps));
+    hipLaunchKernelGGL((k_affine_map<3, false>), dim3(ntiles), dim3(SIM_TILE), 0, st,
+                       static_cast<const sim_replica_t*>(reps),
+                       reinterpret_cast<const SimTile*>(tiles), reinterpret_cast<double4*>(pos4),
+                       nullptr, reinterpret_cast<double4*>(pos_last));
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }

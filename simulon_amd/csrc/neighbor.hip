@@ -59,6 +59,7 @@ __global__ void k_grid(int R, sim_replica_t* reps, const int* flag) {
     P.ncell[1] = nc[1];
     P.ncell[2] = nc[2];
     P.mu_acc = 1.0;
+    for (int k = 0; k < 9; ++k) P.hb_inv[k] = P.hinv[k];
 }
 
 __global__ void k_cell_clear(const int* cell_rep, int ncells, const int* flag, int* cell_count) {

@@ -138,9 +138,11 @@ __device__ __forceinline__ float u01(uint32_t b) {
 }
 
 // Three N(0,1) draws for (atom, step) by Box-Muller on one Philox block.
+// tag separates independent streams (atoms: the default; barostat: its own).
 __device__ __forceinline__ void gauss3(unsigned long long seed, unsigned long long step,
-                                       uint32_t atom, float& g0, float& g1, float& g2) {
-    u32x4 c{atom, (uint32_t)step, (uint32_t)(step >> 32), 0x5eedu};
+                                       uint32_t atom, float& g0, float& g1, float& g2,
+                                       uint32_t tag = 0x5eedu) {
+    u32x4 c{atom, (uint32_t)step, (uint32_t)(step >> 32), tag};
     u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     float ra = sqrtf(-2.0f * logf(u01(r.x)));
     float rb = sqrtf(-2.0f * logf(u01(r.z)));

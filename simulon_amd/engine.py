@@ -102,6 +102,7 @@ class ReplicaSpec:
     kappa: float = 4.57e-5                # 1/bar (barostat.py:27)
     mu_max: float = 0.05                  # barostat.py:28
     seed: int = 0
+    barostat: str = "berendsen"           # NPT controller: berendsen | anisotropic
 
 
 FAMILIES = ("cell", "rows", "global")
@@ -155,6 +156,7 @@ class ReplicaEngine:
         self._graph = None
         self._graph_virial = None
         self.steps_done = 0
+        self.loading = False        # a tensile loader deforms the box at the start of each step
         self.profile = {"graph_replays": 0, "eager_steps": 0}
 
     # ---- layout --------------------------------------------------------------
@@ -325,7 +327,7 @@ class ReplicaEngine:
         P.target_p, P.tau_p, P.kappa, P.mu_max = s.target_pressure, s.tau_p, s.kappa, s.mu_max
         ens = s.ensemble.upper()
         P.langevin = int(ens in ("NVT", "NPT"))
-        P.barostat = int(ens == "NPT")
+        P.barostat = (2 if s.barostat == "anisotropic" else 1) if ens == "NPT" else 0
         damp = np.float32(1.0)
         for t, m in enumerate(self.masses_amu):
             mi = internal_mass_f32(m)
@@ -527,8 +529,20 @@ class ReplicaEngine:
         return self.partial, self.rep_tiles
 
     def barostat(self):
-        call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
-             ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last), stream_ptr())
+        kinds = {s.barostat for s in self.specs if s.ensemble.upper() == "NPT"}
+        if "berendsen" in kinds:
+            call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
+                 ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last), stream_ptr())
+        if "anisotropic" in kinds:
+            call("sim_aniso_barostat", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
+                 ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last),
+                 self.seed, ptr(self.step_ctr), stream_ptr())
+
+    def deform(self):
+        """Scale the lattice rows of the replicas with loading = 1 by their
+        deform factors, carrying the atoms with the box."""
+        call("sim_box_deform", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
+             ptr(self.pos4), ptr(self.pos_last), stream_ptr())
 
     @property
     def has_barostat(self):
@@ -560,6 +574,8 @@ class ReplicaEngine:
         """One MD step for every replica (md_model.py:103-153)."""
         if virial is None:
             virial = True
+        if self.loading:
+            self.deform()
         self.first_half()
         self.build_list(force=False)
         self.compute_forces(virial=virial or self.has_barostat)

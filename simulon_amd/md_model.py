@@ -4,7 +4,8 @@ BackboneInterface / IntegratorInterface (:9-17), SumBackboneInterface (:19-52)
 and BaseModel (:55-166) keep their names, call signatures and output dicts.
 BaseModel.forward() runs one step of the device engine: first half (B+A),
 skin trigger and gated list rebuild, forces, second half (O+B), float64
-reductions and, with a barostat, the Berendsen update — all stream-ordered,
+reductions and, with a barostat, its device update (Berendsen or
+anisotropic) — all stream-ordered,
 without the reference's per-step host synchronisations. BaseModel.run(n)
 replays a captured HIP graph of that step n times.
 """
@@ -63,15 +64,32 @@ class BaseModel(nn.Module):
         self.sum_bone = sum_bone
         self.Integrator = integrator
         self.molecular = molecular
+        self.engine = molecular.engine
         self.barostat = barostat
         self._first_call = True
         self.profile = {"pair_time": 0.0, "integrate_time": 0.0, "pair_calls": 0,
                         "integrate_calls": 0}
-        self.engine = molecular.engine
         ff = sum_bone.device_force_field()
         if ff is None:
             raise NotImplementedError("the device step runs one LJ or EAM backbone")
         self.engine.force_field = ff
+
+    @property
+    def barostat(self):
+        return self.__dict__.get("_barostat")
+
+    @barostat.setter
+    def barostat(self, b):
+        """Assigning a barostat switches the device controller to it (None:
+        no pressure control), as swapping model.barostat does in the
+        reference's workflows (run_scripts/w_tensile.py:232)."""
+        old = self.__dict__.get("_barostat")
+        if old is not None and old is not b and hasattr(old, "_deactivate"):
+            old._deactivate()
+        self.__dict__["_barostat"] = b
+        if b is not None and hasattr(b, "_activate"):
+            b._activate()
+        self.engine._graph = None
 
     def _kinetic_tensor(self):
         return self.engine.kinetic_tensor(0)
@@ -81,6 +99,8 @@ class BaseModel(nn.Module):
         mol = self.molecular
         if mol.needs_update:
             mol._trigger_and_build(force=True)
+        if eng.loading:
+            eng.deform()
         eng.first_half()
         eng.build_list(force=False)
         eng.compute_forces(virial=True)
@@ -116,8 +136,11 @@ class BaseModel(nn.Module):
             "temperature": (2.0 / 3.0) * ke / (self.molecular.atom_count * self.Integrator.BOLTZMAN),
         }
         if self.barostat is not None:
-            o = _lib.Replica.press.offset
-            out["pressure"] = eng.reps[o:o + 8].view(torch.float64)[0]
+            if hasattr(self.barostat, "pressure_axis_bar"):
+                out["pressure"] = self.barostat.pressure_axis_bar.to(torch.float32)
+            else:
+                o = _lib.Replica.press.offset
+                out["pressure"] = eng.reps[o:o + 8].view(torch.float64)[0]
         return out
 
     def run(self, nsteps, graph=True):

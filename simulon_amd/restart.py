@@ -7,7 +7,7 @@ order. Under the extra key "simulon_amd" it adds what the device step needs to
 continue the same trajectory: the float64 coordinates, the Langevin noise seed
 and device step counter (the noise of an atom is keyed by its caller index and
 that counter, so it continues unchanged), the virial tensor, the last
-pressure and the reduced thermo row.
+pressure, the anisotropic barostat's strain rates and the reduced thermo row.
 
 load_checkpoint(model, path) restores them, forces a list rebuild at the next
 step as the reference does (restart.py:111-118), skips the first force
@@ -31,6 +31,7 @@ def save_checkpoint(model, step: int, path: str | Path) -> None:
     eng = model.engine
     th = eng.thermo[0]
     o = _lib.Replica.press.offset
+    oe = _lib.Replica.eta_dot.offset
     ckpt = {
         "step": int(step),
         "coordinates": mol.coordinates.detach().cpu(),
@@ -47,6 +48,7 @@ def save_checkpoint(model, step: int, path: str | Path) -> None:
             "step_counter": eng.step_ctr.detach().cpu().clone(),
             "thermo": th.detach().cpu().clone(),
             "pressure": eng.reps[o:o + 8].view(torch.float64).detach().cpu().clone(),
+            "eta_dot": eng.reps[oe:oe + 24].view(torch.float64).detach().cpu().clone(),
         },
     }
     if torch.cuda.is_available():
@@ -85,6 +87,12 @@ def load_checkpoint(model, path: str | Path) -> int:
     if "pressure" in ext:
         o = _lib.Replica.press.offset
         eng.reps[o:o + 8].view(torch.float64).copy_(ext["pressure"].to(dev))
+    if "eta_dot" in ext:                  # anisotropic barostat strain rates
+        oe = _lib.Replica.eta_dot.offset
+        eng.reps[oe:oe + 24].view(torch.float64).copy_(ext["eta_dot"].to(dev))
+        b = getattr(eng, "_active_barostat", None)
+        if b is not None and hasattr(b, "_eta"):
+            b._eta = ext["eta_dot"].tolist()
     if "rng_cpu" in ckpt:
         torch.set_rng_state(ckpt["rng_cpu"])
     if "rng_cuda" in ckpt and torch.cuda.is_available():

@@ -20,12 +20,17 @@ def _model(ensemble, seed):
                          device="cuda", parameter={"[0 0]": {"epsilon": 0.0104, "sigma": 3.405}},
                          skin_thickness=1.0)
     ff = LennardJonesForce(mol)
-    kw = dict(temperature=(94.4, 94.4), gamma=10.0) if ensemble == "NVT" else {}
+    kw = dict(temperature=(94.4, 94.4), gamma=10.0) if ensemble != "NVE" else {}
     integ = VerletIntegrator(mol, dt=0.002, ensemble=ensemble, seed=seed, **kw)
-    return mol, BaseModel(SumBackboneInterface([ff], mol), integ, mol)
+    baro = None
+    if ensemble == "NPT":           # anisotropic: its strain rates travel in the file
+        from simulon_amd.barostat import AnisotropicNPTBarostat
+        baro = AnisotropicNPTBarostat(mol, target_pressure_bar=[0.0, 50.0, -50.0],
+                                      temperature_k=94.4, compressibility_bar_inv=3.2e-6)
+    return mol, BaseModel(SumBackboneInterface([ff], mol), integ, mol, barostat=baro)
 
 
-@pytest.mark.parametrize("ensemble", ["NVE", "NVT"])
+@pytest.mark.parametrize("ensemble", ["NVE", "NVT", "NPT"])
 def test_restart_continues_trajectory(ensemble, tmp_path):
     from simulon_amd.restart import load_checkpoint, save_checkpoint
     print(f"[case] restart {ensemble}", flush=True)
@@ -49,6 +54,11 @@ def test_restart_continues_trajectory(ensemble, tmp_path):
     assert torch.equal(mol_b.atom_velocities.cpu(), v_saved)
     out_b = model_b.run(40)
     x_b = model_b.engine.to_caller(model_b.engine.positions(torch.float64, unwrap=True)).cpu()
+    if ensemble == "NPT":
+        ea, eb = model_a.barostat.eta_dot.cpu(), model_b.barostat.eta_dot.cpu()
+        print(f"NPT eta_dot {ea.tolist()} vs {eb.tolist()}", flush=True)
+        assert float(ea.abs().max()) > 0 and torch.allclose(ea, eb, rtol=1e-5, atol=1e-9)
+        assert torch.allclose(mol_a.box.H.cpu(), mol_b.box.H.cpu(), rtol=1e-9, atol=0)
     d = float((x_a - x_b).norm(dim=1).max())
     dE = abs(float(out_a["energy"]) - float(out_b["energy"]))
     print(f"{ensemble}: max |dx| after 40 steps {d:.2e} A, |dE| {dE:.2e} eV", flush=True)
