@@ -93,6 +93,54 @@ __device__ __forceinline__ int nbr_at(const int* __restrict__ nbr, int ns, int i
     return k < cnt ? j : i;
 }
 
+// Wave-uniform extrema of an int over the 64 lanes (all lanes active).
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Raw buffer view of device memory (gfx9 descriptor, 32-bit data format):
+// loads take a 32-bit per-lane byte offset from a base held in scalar
+// registers, so no per-lane 64-bit address arithmetic is spent; bytes is the
+// range checked by the hardware (< 2^32).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// Entry k of atom i's column-major list, or i itself at or past the lane's
+// end cnt. k and cmax (the longest list of the wave, <= max_nbr) are
+// wave-uniform: the row is clamped to cmax - 1 by a scalar operation and its
+// base goes into the buffer descriptor, the lane's offset i*4 is loop
+// invariant; rows between the lane's end and cmax, stale, are discarded by
+// the select.
+__device__ __forceinline__ int nbr_row(const int* __restrict__ nbr, int ns, int i, int k, int cnt,
+                                       int cmax) {
+    const int kk = min(k, max(cmax - 1, 0));
+    const int j = __builtin_amdgcn_raw_buffer_load_b32(
+        buf_rsrc(nbr + (size_t)kk * ns, (unsigned)ns * 4u), i * 4, 0, 0);
+    return k < cnt ? j : i;
+}
+
+// Record j of a 16-byte record array (j < 2^28: 32-bit byte offsets).
+__device__ __forceinline__ int4 rec_at(const int4* __restrict__ p, int j) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(buf_rsrc(p, 0xFFFFFFFFu), j * 16, 0, 0);
+    return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+
+// Square root of a squared pair distance with the hardware instruction
+// (within 1 ulp) instead of the correctly rounded expansion, about ten
+// instructions more per pair; r only selects a table interval and its
+// fraction, and enters forces through them.
+__device__ __forceinline__ float pair_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ float sq3(float x, float y, float z) {
     return __builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x));
 }

@@ -148,13 +148,13 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
     Lerp L[UB];
     float4 v[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) xj[u] = pos[j[u]];
+    for (int u = 0; u < UB; ++u) xj[u] = rec_at(pos, j[u]);
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         float rx, ry, rz;
         qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
         r2[u] = sq3(rx, ry, rz);
-        L[u] = lerp_r(__builtin_sqrtf(r2[u]), E);
+        L[u] = lerp_r(pair_sqrt(r2[u]), E);
     }
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
@@ -171,32 +171,33 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
     }
 }
 
+// The wave walks its lists in lock step: k is wave-uniform, batches end at
+// cmax (the wave's longest list) and are masked once k + U passes cmin (its
+// shortest). Masked entries add +0.0f, so the sums are those of each lane's
+// own list in order.
 template <int NT, bool LT, bool ORTHO>
 __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const int4 xi, int cnt,
-                                             const QBox& B, const int4* __restrict__ pos,
+                                             int cmin, int cmax, const QBox& B,
+                                             const int4* __restrict__ pos,
                                              const int* __restrict__ nbr, int ns,
                                              const sim_eam_params_t& E, const float2* tl,
                                              float thr) {
-    constexpr int U = LT ? SIM_EAM_LDS_BATCH : SIM_LIST_BATCH, U2 = SIM_NB_BATCH;
+    constexpr int U = LT ? SIM_EAM_LDS_BATCH : SIM_LIST_BATCH;
     const int ti = NT == 1 ? 0 : xi.w;
-    int k = 0;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
-    for (; k + U <= cnt; k += U) {
+    for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, u, cnt, cmax);
+    for (int k = 0; k < cmax; k += U) {
         int j[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
-        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-        density_batch<U, false, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
-    }
-    for (; k < cnt; k += U2) {   // the rest in small masked batches
-        int j[U2];
-#pragma unroll
-        for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
-        density_batch<U2, true, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
+        for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, k + U + u, cnt, cmax);
+        if (k + U <= cmin)
+            density_batch<U, false, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
+        else
+            density_batch<U, true, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
     }
 }
 
@@ -219,16 +220,19 @@ __global__ void __launch_bounds__(SIM_TILE * TPW)
         const int i = t.start + lt;
         const bool active = have && i < P.atom_base + P.natoms;
         double etot = 0.0;
+        const int cnt = active ? nnbr[i] : 0;
+        const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
         if (active) {
             const QBox B = load_qbox(P);
             const int4 xi = pos[i];
             const int ti = xi.w;
-            const int cnt = nnbr[i];
             float rho = 0.f, ep = 0.f;
             if (P.ortho)
-                density_atom<NT, LT, true>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, tl, thr);
+                density_atom<NT, LT, true>(rho, ep, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, E,
+                                           tl, thr);
             else
-                density_atom<NT, LT, false>(rho, ep, i, xi, cnt, B, pos, nbr, n_stride, E, tl, thr);
+                density_atom<NT, LT, false>(rho, ep, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride,
+                                            E, tl, thr);
             // embedding, eam_force_cu.py:448-459
             const float lo = E.rho_lo[ti];
             const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
@@ -290,14 +294,14 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     Lerp L[UB];
     float4 va[UB], vb[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) xj[u] = rec[j[u]];
+    for (int u = 0; u < UB; ++u) xj[u] = rec_at(rec, j[u]);
 #pragma unroll
-    for (int u = 0; u < UB; ++u) tj[u] = NT == 1 ? 0 : typ[j[u]].w;
+    for (int u = 0; u < UB; ++u) tj[u] = NT == 1 ? 0 : rec_at(typ, j[u]).w;
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         qdisp<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
         r2[u] = sq3(rx[u], ry[u], rz[u]);
-        r[u] = __builtin_sqrtf(r2[u]);
+        r[u] = pair_sqrt(r2[u]);
         L[u] = lerp_r(r[u], E);
     }
 #pragma unroll
@@ -326,33 +330,31 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
 
 // rec: (s, F') records of pass 1; typ: (s, type) records, read for the
 // neighbour types only when there are several (NT == 0).
+// Lock-step walk as in density_atom.
 template <int NT, bool LT, bool VIRIAL, bool ORTHO>
 __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, int ti, int cnt,
-                                           const QBox& B, const int4* __restrict__ rec,
+                                           int cmin, int cmax, const QBox& B,
+                                           const int4* __restrict__ rec,
                                            const int4* __restrict__ typ, const int* __restrict__ nbr,
                                            int ns, const sim_eam_params_t& E, const float2* tl,
                                            float thr) {
-    constexpr int U = LT ? SIM_EAM_FORCE_LDS_BATCH : SIM_EAM_FORCE_BATCH, U2 = SIM_NB_BATCH;
+    constexpr int U = LT ? SIM_EAM_FORCE_LDS_BATCH : SIM_EAM_FORCE_BATCH;
     const float fpi = __int_as_float(xi.w);
-    int k = 0;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
-    for (; k + U <= cnt; k += U) {
+    for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, u, cnt, cmax);
+    for (int k = 0; k < cmax; k += U) {
         int j[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
-        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
-        force_batch<U, false, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
-                                                     tl, thr);
-    }
-    for (; k < cnt; k += U2) {   // the rest in small masked batches
-        int j[U2];
-#pragma unroll
-        for (int u = 0; u < U2; ++u) j[u] = nbr_at(nbr, ns, i, k + u, cnt);
-        force_batch<U2, true, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
-                                                     tl, thr);
+        for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, k + U + u, cnt, cmax);
+        if (k + U <= cmin)
+            force_batch<U, false, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
+                                                         tl, thr);
+        else
+            force_batch<U, true, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
+                                                        tl, thr);
     }
 }
 
@@ -375,17 +377,18 @@ __global__ void __launch_bounds__(SIM_TILE * TPW)
         const int i = t.start + lt;
         const bool active = have && i < P.atom_base + P.natoms;
         ForceAcc a;
+        const int cnt = active ? nnbr[i] : 0;
+        const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
         if (active) {
             const QBox B = load_qbox(P);
             const int4 xi = rec[i];
             const int ti = NT == 1 ? 0 : typ[i].w;
-            const int cnt = nnbr[i];
             if (P.ortho)
-                force_atom<NT, LT, VIRIAL, true>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E,
-                                                 tl, thr);
+                force_atom<NT, LT, VIRIAL, true>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
+                                                 n_stride, E, tl, thr);
             else
-                force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, B, rec, typ, nbr, n_stride, E,
-                                                  tl, thr);
+                force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
+                                                  n_stride, E, tl, thr);
             force[(size_t)i * 3 + 0] = a.fx;
             force[(size_t)i * 3 + 1] = a.fy;
             force[(size_t)i * 3 + 2] = a.fz;

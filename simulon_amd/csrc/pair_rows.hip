@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(SIM_ROW_THREADS)
                         const float rx = (float)(xi - px), ry = (float)(yi - py),
                                     rz = (float)(zi - pz);
                         r2[u] = sq3(rx, ry, rz);
-                        Lt[u] = lerp_idx(__builtin_sqrtf(r2[u]), E);
+                        Lt[u] = lerp_idx(pair_sqrt(r2[u]), E);
                         v[u] = T1[Lt[u].idx];   // one element type (checked at launch)
                     }
 #pragma unroll
@@ -464,12 +464,12 @@ __global__ void __launch_bounds__(SIM_ROW_THREADS)
                         rz[u] = (float)(zi - pz);
                         r2[u] = sq3(rx[u], ry[u], rz[u]);
                         if (!(k + u * LANES < cnt)) r2[u] = 3.0e38f;   // masked: fails r2 < thr
-                        Lt[u] = lerp_idx(__builtin_sqrtf(r2[u]), E);
+                        Lt[u] = lerp_idx(pair_sqrt(r2[u]), E);
                         va[u] = T2[Lt[u].idx];   // one element type (checked at launch)
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        const float r = __builtin_sqrtf(r2[u]);
+                        const float r = pair_sqrt(r2[u]);
                         const float w = r2[u] < thr ? 1.0f : 0.0f;
                         const float dd = lp(va[u].x, va[u].y, Lt[u].frac);
                         const float s = w * (fpi * dd + fpj[u] * dd + lp(va[u].z, va[u].w, Lt[u].frac));

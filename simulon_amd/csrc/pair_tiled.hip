@@ -224,7 +224,7 @@ __device__ __forceinline__ double eam_density_home(const Halo<CAP>& H,
                     float rx, ry, rz, aj;
                     disp_slot<FROM_LDS, 0>(H, pos, nullptr, xi, yi, zi, s, rx, ry, rz, aj);
                     r2[u] = sq3(rx, ry, rz);
-                    L[u] = lerp_idx(__builtin_sqrtf(r2[u]), E);
+                    L[u] = lerp_idx(pair_sqrt(r2[u]), E);
                     const int tj = nt == 1 ? 0 : (int)aj;
                     v[u] = T1[(size_t)(ti * nt + tj) * E.n_r + L[u].idx];
                 }
@@ -240,7 +240,7 @@ __device__ __forceinline__ double eam_density_home(const Halo<CAP>& H,
                 float rx, ry, rz, aj;
                 disp_slot<FROM_LDS, 0>(H, pos, nullptr, xi, yi, zi, s, rx, ry, rz, aj);
                 const float r2 = sq3(rx, ry, rz);
-                const LerpT L = lerp_idx(__builtin_sqrtf(r2), E);
+                const LerpT L = lerp_idx(pair_sqrt(r2), E);
                 const int tj = nt == 1 ? 0 : (int)aj;
                 const float4 v = T1[(size_t)(ti * nt + tj) * E.n_r + L.idx];
                 const float w = r2 < thr ? 1.0f : 0.0f;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void eam_force_home(double (&acc)[7], const Halo<CAP>
             int k = l;
             auto pair = [&](float rx, float ry, float rz, float r2, const LerpT& L,
                             const float4& va, float fpj) {
-                const float r = __builtin_sqrtf(r2);
+                const float r = pair_sqrt(r2);
                 const float w = r2 < thr ? 1.0f : 0.0f;
                 const float dd = lp(va.x, va.y, L.frac);
                 const float s = w * (fpi * dd + fpj * dd + lp(va.z, va.w, L.frac));
@@ -352,7 +352,7 @@ __device__ __forceinline__ void eam_force_home(double (&acc)[7], const Halo<CAP>
                     const int s = flat_slot(H, row[k + u * LANES]);
                     disp_slot<FROM_LDS, 1>(H, pos, fp, xi, yi, zi, s, rx[u], ry[u], rz[u], fpj[u]);
                     r2[u] = sq3(rx[u], ry[u], rz[u]);
-                    L[u] = lerp_idx(__builtin_sqrtf(r2[u]), E);
+                    L[u] = lerp_idx(pair_sqrt(r2[u]), E);
                     va[u] = T2[L[u].idx];   // one element type (checked at launch)
                 }
 #pragma unroll
@@ -363,7 +363,7 @@ __device__ __forceinline__ void eam_force_home(double (&acc)[7], const Halo<CAP>
                 float rx, ry, rz, fpj;
                 disp_slot<FROM_LDS, 1>(H, pos, fp, xi, yi, zi, s, rx, ry, rz, fpj);
                 const float r2 = sq3(rx, ry, rz);
-                const LerpT L = lerp_idx(__builtin_sqrtf(r2), E);
+                const LerpT L = lerp_idx(pair_sqrt(r2), E);
                 pair(rx, ry, rz, r2, L, T2[L.idx], fpj);
             }
         }
