@@ -102,7 +102,7 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
                                          float thr) {
     int4 xj[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) xj[u] = pos[j[u]];
+    for (int u = 0; u < UB; ++u) xj[u] = rec_at(pos, j[u]);
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         float rx, ry, rz;
@@ -119,21 +119,24 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
 #endif
 
 template <bool VIRIAL, int MODE, bool NT1, bool ORTHO>
-__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, const QBox& B,
-                                        const int4* __restrict__ pos, const int* __restrict__ nbr,
-                                        int n_stride, const sim_lj_params_t& lj, float thr) {
+// The wave walks its lists in lock step (k wave-uniform, up to cmax, the
+// wave's longest list); entries past a lane's own end are masked.
+__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, int cmax,
+                                        const QBox& B, const int4* __restrict__ pos,
+                                        const int* __restrict__ nbr, int n_stride,
+                                        const sim_lj_params_t& lj, float thr) {
     constexpr int U = SIM_LJ_BATCH;
     const LJRow<NT1> row(lj, xi.w);
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, u, cnt);
-    for (int k = 0; k < cnt; k += U) {
+    for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, n_stride, i, u, cnt, cmax);
+    for (int k = 0; k < cmax; k += U) {
         int j[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
         // indices of the next batch load while this batch computes
 #pragma unroll
-        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, n_stride, i, k + U + u, cnt);
+        for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, n_stride, i, k + U + u, cnt, cmax);
         lj_batch<U, true, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
     }
 }
@@ -151,14 +154,15 @@ __global__ void __launch_bounds__(SIM_TILE)
     const int i = t.start + threadIdx.x;
     const bool active = i < P.atom_base + P.natoms;
     LJAcc a;
+    const int cnt = active ? nnbr[i] : 0;
+    const int cmax = wave_max_int(cnt);
     if (active) {
         const QBox B = load_qbox(P);
         const int4 xi = pos[i];
-        const int cnt = nnbr[i];
         if (P.ortho)
-            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, cmax, B, pos, nbr, n_stride, lj, thr);
         else
-            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, cmax, B, pos, nbr, n_stride, lj, thr);
         force[(size_t)i * 3 + 0] = a.fx;
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
