@@ -93,6 +93,26 @@ __device__ __forceinline__ int nbr_at(const int* __restrict__ nbr, int ns, int i
     return k < cnt ? j : i;
 }
 
+// block_sum of one 256-thread group g of a larger workgroup (same order, so
+// the same value as block_sum); lds holds [groups][4 * NV]; every thread of
+// the workgroup must call it.
+template <int NV>
+__device__ __forceinline__ void group_sum(double (&v)[NV], double* lds, int g, int lt) {
+    const int lane = lt & 63, wid = lt >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[(g * 4 + wid) * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (lt == 0) {
+        const double* p = lds + g * 4 * NV;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = (p[k] + p[NV + k]) + (p[2 * NV + k] + p[3 * NV + k]);
+    }
+}
+
 // Wave-uniform extrema of an int over the 64 lanes (all lanes active).
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll

@@ -94,26 +94,6 @@ struct TileWalk {
     }
 };
 
-// block_sum of one 256-thread group g of a larger workgroup (same order, so
-// the same value as block_sum); lds holds [groups][4 * NV]; every thread of
-// the workgroup must call it.
-template <int NV>
-__device__ __forceinline__ void group_sum(double (&v)[NV], double* lds, int g, int lt) {
-    const int lane = lt & 63, wid = lt >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) lds[(g * 4 + wid) * NV + k] = v[k];
-    }
-    __syncthreads();
-    if (lt == 0) {
-        const double* p = lds + g * 4 * NV;
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = (p[k] + p[NV + k]) + (p[2 * NV + k] + p[3 * NV + k]);
-    }
-}
-
 // Tile groups per workgroup of the persistent density pass: one table copy
 // in LDS serves them all, so more waves per CU share it.
 #ifndef SIM_EAM_TPW

@@ -141,18 +141,27 @@ __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt,
     }
 }
 
+// Tiles per workgroup. Adjacent tiles share most neighbours, but four per
+// workgroup (one CU's L1 serving them) measured slower than one (1 M atoms:
+// 0.184 vs 0.164 ms).
+#ifndef SIM_LJ_TPW
+#define SIM_LJ_TPW 1
+#endif
+
 template <bool VIRIAL, int MODE, bool NT1>
-__global__ void __launch_bounds__(SIM_TILE)
-    k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
+__global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
+    k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles, int ntiles,
          const int4* __restrict__ pos, const int* __restrict__ nbr,
          const int* __restrict__ nnbr, int n_stride, sim_lj_params_t lj,
          float* __restrict__ force, double* __restrict__ partial, float thr) {
-    __shared__ double lds[4 * 8];
-    const int tb = xcd_block(blockIdx.x, gridDim.x);
-    const SimTile t = tiles[tb];
+    __shared__ double lds[SIM_LJ_TPW * 4 * 8];
+    const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
+    const int tb = xcd_block(blockIdx.x, gridDim.x) * SIM_LJ_TPW + g;
+    const bool have = tb < ntiles;
+    const SimTile t = tiles[have ? tb : 0];
     const sim_replica_t& P = reps[t.rep];
-    const int i = t.start + threadIdx.x;
-    const bool active = i < P.atom_base + P.natoms;
+    const int i = t.start + lt;
+    const bool active = have && i < P.atom_base + P.natoms;
     LJAcc a;
     const int cnt = active ? nnbr[i] : 0;
     const int cmax = wave_max_int(cnt);
@@ -169,8 +178,8 @@ __global__ void __launch_bounds__(SIM_TILE)
     }
     double v[8] = {0.5 * a.e, 0.5 * (a.wxx + a.wyy + a.wzz), 0.5 * a.wxx, 0.5 * a.wyy,
                    0.5 * a.wzz, 0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
-    block_sum<8>(v, lds);
-    if (threadIdx.x == 0) {
+    group_sum<8>(v, lds, g, lt);
+    if (have && lt == 0) {
         double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 8; ++k) p[k] = v[k];
@@ -179,17 +188,18 @@ __global__ void __launch_bounds__(SIM_TILE)
 
 template <bool V, bool NT1>
 void launch_mode(int mode, dim3 g, hipStream_t st, const sim_replica_t* reps, const SimTile* tiles,
-                 const int4* pos, const int* nbr, const int* nnbr, int ns, sim_lj_params_t lj,
-                 float* f, double* part, float thr) {
+                 int ntiles, const int4* pos, const int* nbr, const int* nnbr, int ns,
+                 sim_lj_params_t lj, float* f, double* part, float thr) {
+    const dim3 b(SIM_TILE * SIM_LJ_TPW);
     if (mode == 1)
-        hipLaunchKernelGGL((k_lj<V, 1, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
-                           ns, lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 1, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_lj<V, 2, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
-                           ns, lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 2, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
     else
-        hipLaunchKernelGGL((k_lj<V, 0, NT1>), g, dim3(SIM_TILE), 0, st, reps, tiles, pos, nbr, nnbr,
-                           ns, lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 0, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
+                           lj, f, part, thr);
 }
 
 }  // namespace
@@ -206,11 +216,12 @@ extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntile
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
     auto* X = reinterpret_cast<const int4*>(posq);
-    const dim3 g(ntiles);
+    const dim3 g((ntiles + SIM_LJ_TPW - 1) / SIM_LJ_TPW);
     hipStream_t st = as_stream(stream);
     auto go = [&](auto v, auto one) {
-        launch_mode<decltype(v)::value, decltype(one)::value>(lj.mode, g, st, P, T, X, nbr, nnbr,
-                                                              n_stride, lj, force, partial, thr);
+        launch_mode<decltype(v)::value, decltype(one)::value>(lj.mode, g, st, P, T, ntiles, X, nbr,
+                                                              nnbr, n_stride, lj, force, partial,
+                                                              thr);
     };
     using yes = std::true_type;
     using no = std::false_type;
