@@ -23,6 +23,9 @@
  *   posq   int32 [N][4]  fractional fixed-point records (sim_pos_quantize) read
  *                        by the full-list force kernels
  *
+ * The full-list force kernels address records and list rows by 32-bit byte
+ * offsets: N < 2^28 and n_stride < 2^30.
+ *
  * Reference interfaces replaced (JinZongxiao/Simulon):
  *   sim_nl_build / sim_nl_half_*   simulon_cuda.neighbor_search_cuda
  *        ("cuda source/neighbor_search_kernel.cu:73-138") and the PyTorch

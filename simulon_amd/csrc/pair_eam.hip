@@ -432,7 +432,8 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
                                sim_eam_params_t eam, float* rho, float* fp, int* rec,
                                double* partial, void* stream) {
     (void)R;
-    if (!reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 || !rho ||
+    if (!reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
+        n_stride >= (1 << 28) || !rho ||
         !fp || !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
         eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (const int*)rec == posq)
         return SIM_E_ARG;
@@ -466,6 +467,7 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
                              int virial, void* stream) {
     (void)R;
     if (!reps || !tiles || ntiles <= 0 || !posq || !rec || !nbr || !nnbr || n_stride <= 0 ||
+        n_stride >= (1 << 28) ||
         !force || !partial || !eam.pass2 || eam.n_r < 2 || eam.ntypes < 1 ||
         eam.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;

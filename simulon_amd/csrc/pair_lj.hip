@@ -209,6 +209,7 @@ extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntile
                             sim_lj_params_t lj, float* force, double* partial, int virial,
                             void* stream) {
     if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
+        n_stride >= (1 << 28) ||
         !force || !partial || lj.mode < 0 || lj.mode > 2 || !(lj.rc > 0.f) || lj.ntypes < 1 ||
         lj.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;

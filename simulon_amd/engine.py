@@ -165,8 +165,9 @@ class ReplicaEngine:
         self.natoms = [int(len(s.positions)) for s in self.specs]
         self.atom_base = np.concatenate([[0], np.cumsum(self.natoms)]).astype(np.int64)
         self.N = int(self.atom_base[-1])
-        if self.N >= 2 ** 31 - 1:
-            raise ValueError("too many atoms for int32 indices")
+        if self.N >= 2 ** 28:
+            # the pair kernels address 16-byte records by 32-bit byte offsets
+            raise ValueError("at most 2^28 - 1 atoms per engine")
         tiles, rep_tiles = [], [0]
         for r in range(R):
             for k in range(0, self.natoms[r], _lib.SIM_TILE):
