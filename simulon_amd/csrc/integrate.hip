@@ -141,6 +141,9 @@ __global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2,
 // the other workgroups of its column return at once. Both levels have a fixed
 // order, so the result is deterministic.
 #define SIM_RED_CHUNKS 64
+#ifndef SIM_RED_GX
+#define SIM_RED_GX 8      // blocks per replica in the chunk pass
+#endif
 
 // Rows per chunk of a replica with n rows, and the number of chunks used.
 __device__ __forceinline__ int red_len(int n) {
@@ -152,33 +155,38 @@ __global__ void __launch_bounds__(256)
     k_reduce_chunks(const int* __restrict__ rep_rows, const double* __restrict__ partial, int c0,
                     int c1, double* __restrict__ chunks) {
     __shared__ double lds[4 * NC];
-    const int r = blockIdx.y, ck = blockIdx.x;
+    const int r = blockIdx.y;
     const int b0 = rep_rows[r], e0 = rep_rows[r + 1];
     const int len = red_len(e0 - b0);
-    const int b = b0 + ck * len, e = min(e0, b + len);
-    if (ck > 0 && b >= e) return;   // unused chunk (chunk 0 always writes)
-    double s[NC], comp[NC];
+    // a few blocks per replica walk its chunks (most replicas need one or
+    // two), instead of one block per possible chunk
+    for (int ck = blockIdx.x; ck < SIM_RED_CHUNKS; ck += gridDim.x) {
+        const int b = b0 + ck * len, e = min(e0, b + len);
+        if (ck > 0 && b >= e) break;   // unused chunk (chunk 0 always writes)
+        double s[NC], comp[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) s[c] = comp[c] = 0.0;
-    for (int k = b + (int)threadIdx.x; k < e; k += 256) {
-        const double* row = partial + (size_t)k * SIM_NPART;
+        for (int c = 0; c < NC; ++c) s[c] = comp[c] = 0.0;
+        for (int k = b + (int)threadIdx.x; k < e; k += 256) {
+            const double* row = partial + (size_t)k * SIM_NPART;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const double x = c0 + c < c1 ? row[c0 + c] : 0.0;
-            const double tt = s[c] + x;
-            comp[c] += fabs(s[c]) >= fabs(x) ? (s[c] - tt) + x : (x - tt) + s[c];
-            s[c] = tt;
+            for (int c = 0; c < NC; ++c) {
+                const double x = c0 + c < c1 ? row[c0 + c] : 0.0;
+                const double tt = s[c] + x;
+                comp[c] += fabs(s[c]) >= fabs(x) ? (s[c] - tt) + x : (x - tt) + s[c];
+                s[c] = tt;
+            }
         }
-    }
-    double v[NC];
+        double v[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = s[c] + comp[c];
-    block_sum<NC>(v, lds);
-    if (threadIdx.x == 0) {
-        double* out = chunks + ((size_t)r * SIM_RED_CHUNKS + ck) * SIM_NPART;
+        for (int c = 0; c < NC; ++c) v[c] = s[c] + comp[c];
+        block_sum<NC>(v, lds);
+        if (threadIdx.x == 0) {
+            double* out = chunks + ((size_t)r * SIM_RED_CHUNKS + ck) * SIM_NPART;
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-            if (c0 + c < c1) out[c0 + c] = v[c];
+            for (int c = 0; c < NC; ++c)
+                if (c0 + c < c1) out[c0 + c] = v[c];
+        }
+        __syncthreads();   // lds is reused by the next chunk
     }
 }
 
@@ -521,7 +529,7 @@ extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int
         col0 >= col1)
         return SIM_E_ARG;
     hipStream_t st = as_stream(stream);
-    const dim3 g(SIM_RED_CHUNKS, R);
+    const dim3 g(SIM_RED_GX, R);
     if (col1 - col0 <= 8)
         hipLaunchKernelGGL(k_reduce_chunks<8>, g, dim3(256), 0, st, rep_rows, partial, col0, col1,
                            chunks);
