@@ -278,7 +278,7 @@ def bench_replicas(args):
     specs = replica_specs(args.replicas)
     natoms = np.array([len(s.positions) for s in specs])
     log(f"replicas {len(specs)} atoms total {natoms.sum()} min {natoms.min()} max {natoms.max()}")
-    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024,
+    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=args.skin, seed=2024,
                         family=args.family)
     eng.force_field = EAMForceField(setfl, device=eng.device)
     eng.setup()
@@ -296,7 +296,7 @@ def bench_replicas(args):
     plain_t = 0.0
     plain_steps = args.plain_steps or args.steps
     for k, s in enumerate(specs):
-        e1 = ReplicaEngine([s], [lat.W_MASS], cutoff=5.5, skin=1.0, seed=2024 + k)
+        e1 = ReplicaEngine([s], [lat.W_MASS], cutoff=5.5, skin=args.skin, seed=2024 + k)
         e1.force_field = eng.force_field
         e1.setup()
         e1.run(2, graph=True, virial=False)
@@ -317,7 +317,7 @@ def bench_replicas(args):
         "config": {"workload": "w_eam_replicas", "replicas": len(specs),
                    "atoms_total": int(natoms.sum()), "atoms_min": int(natoms.min()),
                    "atoms_max": int(natoms.max()), "dt_fs": 1.0, "temperatures_K": [100, 1500],
-                   "gamma_per_ps": 10.0, "cutoff": 5.5, "skin": 1.0, "size_seed": 3,
+                   "gamma_per_ps": 10.0, "cutoff": 5.5, "skin": args.skin, "size_seed": 3,
                    "eam_table": "synthetic setfl, 5000 points"},
         "plain_loop_atom_steps_per_s": round(float(plain), 1),
         "speedup_vs_plain": round(float(batched / plain), 3),
@@ -536,6 +536,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="w_eam_replicas")
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--skin", type=float, default=1.0,
+                    help="Verlet skin of the replica workload, Angstrom")
     ap.add_argument("--plain-steps", type=int, default=0)
     ap.add_argument("--fcc-cells", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
