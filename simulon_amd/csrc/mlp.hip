@@ -147,22 +147,22 @@ __device__ __forceinline__ void cutoff_fn(float r, const Basis& b, float& fc, fl
 // records (pair_common.h).
 #define SIM_MLP_BATCH 8
 template <bool ORTHO, typename Body>
-__device__ __forceinline__ void walk(int i, const int4 xi, int cnt, const QBox& B,
+__device__ __forceinline__ void walk(int i, const int4 xi, int cnt, int cmax, const QBox& B,
                                      const int4* __restrict__ pos, const int* __restrict__ nbr,
                                      int ns, Body&& body) {
     constexpr int U = SIM_MLP_BATCH;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, u, cnt);
-    for (int k = 0; k < cnt; k += U) {
+    for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, u, cnt, cmax);
+    for (int k = 0; k < cmax; k += U) {   // wave-uniform k, as in the pair kernels
         int j[U];
         int4 xj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) j[u] = jn[u];
 #pragma unroll
-        for (int u = 0; u < U; ++u) jn[u] = nbr_at(nbr, ns, i, k + U + u, cnt);
+        for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, k + U + u, cnt, cmax);
 #pragma unroll
-        for (int u = 0; u < U; ++u) xj[u] = pos[j[u]];
+        for (int u = 0; u < U; ++u) xj[u] = rec_at(pos, j[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (k + u < cnt) {
@@ -183,13 +183,13 @@ __device__ __forceinline__ int lane_atom(const SimTile& t) {
 // ---- energy: descriptors, network, input gradient -------------------------
 
 template <bool ORTHO>
-__device__ __forceinline__ void descriptors(float (&g)[8], int i, const sim_replica_t& P,
+__device__ __forceinline__ void descriptors(float (&g)[8], int i, int cmax, const sim_replica_t& P,
                                             const int4* __restrict__ pos,
                                             const int* __restrict__ nbr,
                                             const int* __restrict__ nnbr, int ns, const Basis& b) {
     const QBox B = load_qbox(P);
     const int4 xi = pos[i];
-    walk<ORTHO>(i, xi, nnbr[i], B, pos, nbr, ns, [&](int, float rx, float ry, float rz) {
+    walk<ORTHO>(i, xi, nnbr[i], cmax, B, pos, nbr, ns, [&](int, float rx, float ry, float rz) {
         const float r2 = sq3(rx, ry, rz);
         if (r2 < b.thr) {
             const float r = sqrtf(r2);
@@ -219,12 +219,13 @@ __global__ void __launch_bounds__(256)
     const int i = lane_atom(t);
     const bool active = i < P.atom_base + P.natoms;
     const Basis b = make_basis(net, q, thr);
+    const int cmax = wave_max_int(active ? nnbr[i] : 0);
     float g[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         if (P.ortho)
-            descriptors<true>(g, i, P, pos, nbr, nnbr, ns, b);
+            descriptors<true>(g, i, cmax, P, pos, nbr, nnbr, ns, b);
         else
-            descriptors<false>(g, i, P, pos, nbr, nnbr, ns, b);
+            descriptors<false>(g, i, cmax, P, pos, nbr, nnbr, ns, b);
     }
     const int f0 = 4 * q, f1 = 16 + 4 * q;
     f4v x0, x1;
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(256)
 // ---- forces ------------------------------------------------------------------
 
 template <bool VIRIAL, bool ORTHO>
-__device__ __forceinline__ void force_atom(float (&acc)[10], int i, const sim_replica_t& P,
+__device__ __forceinline__ void force_atom(float (&acc)[10], int i, int cmax, const sim_replica_t& P,
                                            const int4* __restrict__ pos,
                                            const int* __restrict__ nbr,
                                            const int* __restrict__ nnbr, int ns,
@@ -315,7 +316,7 @@ __device__ __forceinline__ void force_atom(float (&acc)[10], int i, const sim_re
     const int4 xi = pos[i];
     const f4v ci0 = *reinterpret_cast<const f4v*>(c + (size_t)i * ND + 4 * q);
     const f4v ci1 = *reinterpret_cast<const f4v*>(c + (size_t)i * ND + 16 + 4 * q);
-    walk<ORTHO>(i, xi, nnbr[i], B, pos, nbr, ns, [&](int j, float rx, float ry, float rz) {
+    walk<ORTHO>(i, xi, nnbr[i], cmax, B, pos, nbr, ns, [&](int j, float rx, float ry, float rz) {
         const float r2 = sq3(rx, ry, rz);
         if (r2 < b.thr) {
             const f4v cj0 = *reinterpret_cast<const f4v*>(c + (size_t)j * ND + 4 * q);
@@ -364,12 +365,13 @@ __global__ void __launch_bounds__(256)
     const int i = lane_atom(t);
     const bool active = i < P.atom_base + P.natoms;
     const Basis b = make_basis(net, q, thr);
+    const int cmax = wave_max_int(active ? nnbr[i] : 0);
     float acc[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         if (P.ortho)
-            force_atom<VIRIAL, true>(acc, i, P, pos, nbr, nnbr, ns, c, b, q);
+            force_atom<VIRIAL, true>(acc, i, cmax, P, pos, nbr, nnbr, ns, c, b, q);
         else
-            force_atom<VIRIAL, false>(acc, i, P, pos, nbr, nnbr, ns, c, b, q);
+            force_atom<VIRIAL, false>(acc, i, cmax, P, pos, nbr, nnbr, ns, c, b, q);
     }
     float fx = acc[0], fy = acc[1], fz = acc[2];
 #pragma unroll
@@ -399,7 +401,7 @@ __global__ void __launch_bounds__(256)
 // ---- force-matching adjoint ---------------------------------------------------
 
 template <bool ORTHO>
-__device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, const sim_replica_t& P,
+__device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, int cmax, const sim_replica_t& P,
                                              const int4* __restrict__ pos,
                                              const int* __restrict__ nbr,
                                              const int* __restrict__ nnbr, int ns,
@@ -410,7 +412,7 @@ __device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, const sim_rep
     const float dix = F[(size_t)i * 3 + 0] - Fr[(size_t)i * 3 + 0];
     const float diy = F[(size_t)i * 3 + 1] - Fr[(size_t)i * 3 + 1];
     const float diz = F[(size_t)i * 3 + 2] - Fr[(size_t)i * 3 + 2];
-    walk<ORTHO>(i, xi, nnbr[i], B, pos, nbr, ns, [&](int j, float rx, float ry, float rz) {
+    walk<ORTHO>(i, xi, nnbr[i], cmax, B, pos, nbr, ns, [&](int j, float rx, float ry, float rz) {
         const float r2 = sq3(rx, ry, rz);
         if (r2 < b.thr) {
             const float djx = F[(size_t)j * 3 + 0] - Fr[(size_t)j * 3 + 0];
@@ -442,13 +444,15 @@ __global__ void __launch_bounds__(256)
     const sim_replica_t& P = reps[t.rep];
     const int lane = threadIdx.x & 63, q = lane >> 4;
     const int i = lane_atom(t);
-    if (i >= P.atom_base + P.natoms) return;
+    const bool active = i < P.atom_base + P.natoms;
+    const int cmax = wave_max_int(active ? nnbr[i] : 0);
+    if (!active) return;
     const Basis b = make_basis(net, q, thr);
     float u[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (P.ortho)
-        adjoint_atom<true>(u, i, P, pos, nbr, nnbr, ns, F, Fr, b);
+        adjoint_atom<true>(u, i, cmax, P, pos, nbr, nnbr, ns, F, Fr, b);
     else
-        adjoint_atom<false>(u, i, P, pos, nbr, nnbr, ns, F, Fr, b);
+        adjoint_atom<false>(u, i, cmax, P, pos, nbr, nnbr, ns, F, Fr, b);
     const float scale = 2.0f * wF / (3.0f * (float)P.natoms);
     const int f0 = 4 * q, f1 = 16 + 4 * q;
     f4v v0, v1;

@@ -84,14 +84,6 @@ __device__ __forceinline__ void qdisp(const int4& a, const int4& b, const QBox& 
     }
 }
 
-// Entry k of atom i's column-major list, or i itself at or past the end. The
-// load always happens (at a clamped, valid entry), so no branch surrounds it
-// and a batch of them stays in flight together.
-__device__ __forceinline__ int nbr_at(const int* __restrict__ nbr, int ns, int i, int k, int cnt) {
-    const int kk = max(min(k, cnt - 1), 0);
-    const int j = nbr[(size_t)kk * ns + i];
-    return k < cnt ? j : i;
-}
 
 // block_sum of one 256-thread group g of a larger workgroup (same order, so
 // the same value as block_sum); lds holds [groups][4 * NV]; every thread of
