@@ -71,7 +71,8 @@ __global__ void __launch_bounds__(SIM_TILE)
     const int i = t.start + threadIdx.x;
     double ke = 0.0, kxx = 0.0, kyy = 0.0, kzz = 0.0, kxy = 0.0, kxz = 0.0, kyz = 0.0;
     if (i < P.atom_base + P.natoms) {
-        const int ty_i = (int)pos[i].w;
+        // one type: no need to read the position record for it
+        const int ty_i = ty.ntypes == 1 ? 0 : (int)pos[i].w;
         const float m = ty.mass[ty_i];
         const float dt = P.dtf;
         const size_t o = (size_t)i * 3;
