@@ -99,11 +99,12 @@ def kernel_time_ms(launch, reps=50):
     return a.elapsed_time(b) / reps
 
 
-def eam_kernel_roofline(eng, workload="w_eam_replicas"):
+def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     """Roofline of the EAM passes with the algorithmic bytes per atom:
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
     read) + 12 (F write). The dominant (longer) pass is reported; the kernels
-    timed are the ones the step launches (row kernels or global-list)."""
+    timed are the ones the step launches (row kernels or global-list, with the
+    virial only when the step computes it: NPT)."""
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
@@ -120,7 +121,7 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas"):
 
         def p2():
             call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
-                 1, ptr(eng.overflow), stream_ptr())
+                 int(virial), ptr(eng.overflow), stream_ptr())
     else:
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
@@ -133,7 +134,7 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas"):
         def p2():
             call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
                  ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force),
-                 ptr(eng.partial), 1, stream_ptr())
+                 ptr(eng.partial), int(virial), stream_ptr())
 
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
     b1 = eng.N * (12 + 4 * nbar + 4)
@@ -149,21 +150,23 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas"):
             "other_pass_ms": round(t1 if t == t2 else t2, 5)}
 
 
-def lj_kernel_roofline(eng, workload="lj_argon_npt_tri"):
+def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
+    """Roofline of the LJ force kernel the step launches (with the virial
+    only when the step computes it: NPT); bytes per atom 12 + 4 nbar + 12."""
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
 
     if eng.use_tiled:
         def k():
-            ff.compute(eng, True)
+            ff.compute(eng, virial)
     else:
         eng.quantize()
 
         def k():
             call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
                  ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force), ptr(eng.partial),
-                 1, stream_ptr())
+                 int(virial), stream_ptr())
 
     t = kernel_time_ms(k)
     b = eng.N * (12 + 4 * nbar + 12)
@@ -290,7 +293,7 @@ def bench_replicas(args):
     builds = eng.nbuild.cpu().numpy()
     log(f"batched: {t * 1e3 / args.steps:.3f} ms/step, {batched:.4e} atom-steps/s, "
         f"rebuilds per replica min {builds.min()} max {builds.max()}")
-    roof = eam_kernel_roofline(eng)
+    roof = eam_kernel_roofline(eng, virial=False)      # the step above runs without it
     log(f"roofline {roof}")
     # plain per-replica loop on the same GPU
     plain_t = 0.0
@@ -375,8 +378,8 @@ def bench_single(args):
         drift = ((th[0] + th[8]) - e0) / eng.N / (2000 * dt_ps * 1e-3)
         log(f"NVE: E/atom {e0 / eng.N:.6f} -> {(th[0] + th[8]) / eng.N:.6f} eV, drift "
             f"{drift:.3e} eV/atom/ns")
-    roof = lj_kernel_roofline(eng, w) if isinstance(eng.force_field, LJForceField) else \
-        eam_kernel_roofline(eng, w)
+    roof = lj_kernel_roofline(eng, w, virial=virial) \
+        if isinstance(eng.force_field, LJForceField) else eam_kernel_roofline(eng, w, virial=virial)
     cpu = None
     if not args.no_cpu:
         if w == "lj_argon_nvt":
