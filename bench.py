@@ -269,10 +269,17 @@ def cpu_baseline_lj(pos, H, dt, seconds=15.0, mode="raw"):
                                    noise=rng.standard_normal((n, 3)).astype(np.float32))
         steps += 1
     el = time.perf_counter() - t0
+    # the reference's list build (blocked O(N^2) brute force, gpu_kdtree.py:90-156)
+    # on the same atoms, timed once
+    tb = time.perf_counter()
+    onb.brute_force_half_list(x.astype(np.float32), H, onb.list_radius_sq(8.5, 1.0))
+    tb = time.perf_counter() - tb
     return {"value": round(steps * n / el, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle float32 LJ + BAOB step, {n} Ar atoms, {steps} steps in {el:.1f} s, "
                       f"list built once (host {os.cpu_count()} cores visible, numpy "
-                      f"single-threaded)"}
+                      f"single-threaded)",
+            "list_build": {"algorithm": "O(N^2) blocked brute force, the reference's",
+                           "atoms": n, "seconds": round(tb, 3)}}
 
 
 def bench_replicas(args):
