@@ -112,17 +112,17 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
     }
 }
 
-// LJ takes the whole list in masked batches of SIM_LJ_BATCH (measured faster
-// than a large unmasked batch plus a small masked rest, unlike EAM).
+// LJ walks the list in batches of SIM_LJ_BATCH, unmasked while every lane of
+// the wave has entries left, masked after.
 #ifndef SIM_LJ_BATCH
 #define SIM_LJ_BATCH 8
 #endif
 
 template <bool VIRIAL, int MODE, bool NT1, bool ORTHO>
 // The wave walks its lists in lock step (k wave-uniform, up to cmax, the
-// wave's longest list); entries past a lane's own end are masked.
-__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, int cmax,
-                                        const QBox& B, const int4* __restrict__ pos,
+// wave's longest list; batches past cmin, its shortest, are masked).
+__device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, int cmin,
+                                        int cmax, const QBox& B, const int4* __restrict__ pos,
                                         const int* __restrict__ nbr, int n_stride,
                                         const sim_lj_params_t& lj, float thr) {
     constexpr int U = SIM_LJ_BATCH;
@@ -137,7 +137,10 @@ __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt,
         // indices of the next batch load while this batch computes
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, n_stride, i, k + U + u, cnt, cmax);
-        lj_batch<U, true, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
+        if (k + U <= cmin)   // every lane of the wave has these entries
+            lj_batch<U, false, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
+        else
+            lj_batch<U, true, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
     }
 }
 
@@ -164,14 +167,16 @@ __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
     const bool active = have && i < P.atom_base + P.natoms;
     LJAcc a;
     const int cnt = active ? nnbr[i] : 0;
-    const int cmax = wave_max_int(cnt);
+    const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
     if (active) {
         const QBox B = load_qbox(P);
         const int4 xi = pos[i];
         if (P.ortho)
-            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, cmax, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj,
+                                             thr);
         else
-            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, cmax, B, pos, nbr, n_stride, lj, thr);
+            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj,
+                                              thr);
         force[(size_t)i * 3 + 0] = a.fx;
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
