@@ -67,15 +67,23 @@ __device__ __forceinline__ QBox load_qbox(const sim_replica_t& P) {
     return b;
 }
 
-template <bool ORTHO>
+// Box forms of the displacement: 0 general rows, 1 orthogonal, 2 lower
+// triangular rows (a along x, b in the xy plane: h01 = h02 = h12 = 0, the
+// LAMMPS form). Form 2 leaves out products with those zeros, which the
+// general form adds exactly, so both give the same displacement.
+template <int GEO>
 __device__ __forceinline__ void qdisp(const int4& a, const int4& b, const QBox& B, float& rx,
                                       float& ry, float& rz) {
     const double s0 = (double)(int)((unsigned)a.x - (unsigned)b.x);
     const double s1 = (double)(int)((unsigned)a.y - (unsigned)b.y);
     const double s2 = (double)(int)((unsigned)a.z - (unsigned)b.z);
-    if (ORTHO) {
+    if (GEO == 1) {
         rx = (float)(s0 * B.h[0]);
         ry = (float)(s1 * B.h[4]);
+        rz = (float)(s2 * B.h[8]);
+    } else if (GEO == 2) {
+        rx = (float)fma(s2, B.h[6], fma(s1, B.h[3], s0 * B.h[0]));
+        ry = (float)fma(s2, B.h[7], s1 * B.h[4]);
         rz = (float)(s2 * B.h[8]);
     } else {
         rx = (float)fma(s2, B.h[6], fma(s1, B.h[3], s0 * B.h[0]));
@@ -84,6 +92,10 @@ __device__ __forceinline__ void qdisp(const int4& a, const int4& b, const QBox& 
     }
 }
 
+// Box form of a replica for qdisp (uniform per replica).
+__device__ __forceinline__ int box_geo(const sim_replica_t& P) {
+    return P.ortho ? 1 : (P.h[1] == 0.0 && P.h[2] == 0.0 && P.h[5] == 0.0 ? 2 : 0);
+}
 
 // block_sum of one 256-thread group g of a larger workgroup (same order, so
 // the same value as block_sum); lds holds [groups][4 * NV]; every thread of

@@ -95,7 +95,7 @@ struct LJRow {
 
 // One batch of UB list entries from k (indices in j[]); MASKED: entries past
 // the end of the list (self) fail the cutoff test.
-template <int UB, bool MASKED, bool VIRIAL, int MODE, bool NT1, bool ORTHO>
+template <int UB, bool MASKED, bool VIRIAL, int MODE, bool NT1, int GEO>
 __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, int cnt,
                                          const int4 xi, const LJRow<NT1>& row, const QBox& B,
                                          const int4* __restrict__ pos, const sim_lj_params_t& lj,
@@ -106,7 +106,7 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         float rx, ry, rz;
-        qdisp<ORTHO>(xi, xj[u], B, rx, ry, rz);
+        qdisp<GEO>(xi, xj[u], B, rx, ry, rz);
         const float r2 = (!MASKED || k + u < cnt) ? sq3(rx, ry, rz) : thr;
         lj_pair<VIRIAL, MODE>(a, rx, ry, rz, r2, thr, row.e(xj[u].w), row.s(xj[u].w), lj);
     }
@@ -118,7 +118,7 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
 #define SIM_LJ_BATCH 8
 #endif
 
-template <bool VIRIAL, int MODE, bool NT1, bool ORTHO>
+template <bool VIRIAL, int MODE, bool NT1, int GEO>
 // The wave walks its lists in lock step (k wave-uniform, up to cmax, the
 // wave's longest list; batches past cmin, its shortest, are masked).
 __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, int cmin,
@@ -138,9 +138,9 @@ __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt,
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, n_stride, i, k + U + u, cnt, cmax);
         if (k + U <= cmin)   // every lane of the wave has these entries
-            lj_batch<U, false, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
+            lj_batch<U, false, VIRIAL, MODE, NT1, GEO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
         else
-            lj_batch<U, true, VIRIAL, MODE, NT1, ORTHO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
+            lj_batch<U, true, VIRIAL, MODE, NT1, GEO>(a, j, k, cnt, xi, row, B, pos, lj, thr);
     }
 }
 
@@ -171,12 +171,13 @@ __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
     if (active) {
         const QBox B = load_qbox(P);
         const int4 xi = pos[i];
-        if (P.ortho)
-            lj_atom<VIRIAL, MODE, NT1, true>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj,
-                                             thr);
+        const int geo = box_geo(P);
+        if (geo == 1)
+            lj_atom<VIRIAL, MODE, NT1, 1>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj, thr);
+        else if (geo == 2)
+            lj_atom<VIRIAL, MODE, NT1, 2>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj, thr);
         else
-            lj_atom<VIRIAL, MODE, NT1, false>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj,
-                                              thr);
+            lj_atom<VIRIAL, MODE, NT1, 0>(a, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, lj, thr);
         force[(size_t)i * 3 + 0] = a.fx;
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
