@@ -457,10 +457,15 @@ __global__ void __launch_bounds__(SIM_TILE)
 // the 8 lanes, and a write pass. The decision per candidate and the order of
 // every list (stencil cells in (z, y, x) order, atoms in index order) are
 // k_list's (WRAPPED), so the lists are identical. A stencil of more than
-// SIM_LIST_CAP atoms reads its candidates from global memory instead.
+// SIM_LIST_CAP atoms reads its candidates from global memory instead (512
+// measured far slower on W and Ar: many stencils exceed it).
+#ifndef SIM_LIST_CAP
 #define SIM_LIST_CAP 1024
+#endif
 #define SIM_LIST_LANES 8
+#ifndef SIM_LIST_GRID
 #define SIM_LIST_GRID 2048   // persistent workgroups over the rebuilding cells
+#endif
 
 // act[r] = cells before replica r of the rebuilding replicas built here;
 // act[R] = their total. One workgroup of 256 threads.
