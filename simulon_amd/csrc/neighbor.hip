@@ -21,6 +21,8 @@
 //            cells) distinct stencil cells; column-major full list, and with
 //            a reorder also the 16-bit column slots of the row kernels
 //   rows     per-replica prefix of the cell-row counts (ny * nz)
+#include <type_traits>
+
 #include "sim_common.h"
 
 namespace {
@@ -466,6 +468,9 @@ __global__ void __launch_bounds__(SIM_TILE)
 #ifndef SIM_LIST_GRID
 #define SIM_LIST_GRID 2048   // persistent workgroups over the rebuilding cells
 #endif
+#ifndef SIM_LIST_STAGE_F32
+#define SIM_LIST_STAGE_F32 1
+#endif
 
 // act[r] = cells before replica r of the rebuilding replicas built here;
 // act[R] = their total. One workgroup of 256 threads.
@@ -497,20 +502,32 @@ __global__ void __launch_bounds__(256)
     if (tid == 0) act[R] = carry;
 }
 
+// Waves per SIMD the staged build's registers are held to (5: 96 VGPRs, no
+// spills with float staging; measured faster than 4, 6 spills).
+#ifndef SIM_LIST_WAVES
+#define SIM_LIST_WAVES 5
+#endif
 template <typename TC, typename T, int STRIDE>
-__global__ void __launch_bounds__(SIM_TILE)
+__global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
     k_list_staged(int R, const sim_replica_t* reps, const int* act, const T* x,
                   const int* cell_count, const int* cell_start, int* nbr, int* nnbr, int n_stride,
                   int max_nbr, int* overflow, double* pos_last, unsigned short* nbr16,
                   int* stencil_max) {
-    __shared__ TC qx[SIM_LIST_CAP], qy[SIM_LIST_CAP], qz[SIM_LIST_CAP];
+    // staged candidates: float offsets from the home cell's first atom (half
+    // the LDS of float64, so more workgroups per CU); a candidate they place
+    // within 1e-5 of r_l^2 takes the float64 tests below, so the lists stay
+    // those of k_list
+    using QT = typename std::conditional<SIM_LIST_STAGE_F32 != 0, float, TC>::type;
+    __shared__ QT qx[SIM_LIST_CAP], qy[SIM_LIST_CAP], qz[SIM_LIST_CAP];
     __shared__ int qs[SIM_LIST_CAP];
     __shared__ unsigned short q16[SIM_LIST_CAP];
     __shared__ int mstart[27], mcount[27], mpre[28], mslot[27], mcol[3];
     __shared__ TC msh[27][3];
+    __shared__ double mref[3];
     const int tid = threadIdx.x;
     const int total = act[R];
     const double margin = sizeof(TC) == 8 ? 1e-9 : 1e-5;   // as k_list
+    const double smargin = sizeof(QT) == 8 ? 1e-9 : 1e-5;
     for (int ac = blockIdx.x; ac < total; ac += gridDim.x) {
         int lo = 0, hi = R - 1;   // last replica with act[r] <= ac
         while (lo < hi) {
@@ -550,6 +567,8 @@ __global__ void __launch_bounds__(SIM_TILE)
             mcol[0] = coff[0];
             mcol[1] = coff[1];
             mcol[2] = coff[2];
+            for (int c = 0; c < 3; ++c)
+                mref[c] = mcount[13] > 0 ? (double)x[(size_t)mstart[13] * STRIDE + c] : 0.0;
         }
         __syncthreads();
         const int nt = mpre[27];
@@ -565,9 +584,9 @@ __global__ void __launch_bounds__(SIM_TILE)
             for (int f = tid; f < nt; f += SIM_TILE) {
                 const int k = cell_of(f);
                 const int sj = mstart[k] + (f - mpre[k]);
-                qx[f] = (TC)x[(size_t)sj * STRIDE + 0] + msh[k][0];
-                qy[f] = (TC)x[(size_t)sj * STRIDE + 1] + msh[k][1];
-                qz[f] = (TC)x[(size_t)sj * STRIDE + 2] + msh[k][2];
+                qx[f] = (QT)((double)((TC)x[(size_t)sj * STRIDE + 0] + msh[k][0]) - mref[0]);
+                qy[f] = (QT)((double)((TC)x[(size_t)sj * STRIDE + 1] + msh[k][1]) - mref[1]);
+                qz[f] = (QT)((double)((TC)x[(size_t)sj * STRIDE + 2] + msh[k][2]) - mref[2]);
                 qs[f] = sj;
                 q16[f] = (unsigned short)(mslot[k] + (f - mpre[k]));
             }
@@ -576,6 +595,7 @@ __global__ void __launch_bounds__(SIM_TILE)
         const int h0 = mstart[13], nh = mcount[13];   // the cell itself
         const TC rl2 = (TC)P.rl2;
         const TC lo2 = rl2 * (TC)(1.0 - margin), hi2 = rl2 * (TC)(1.0 + margin);
+        const QT slo2 = (QT)(P.rl2 * (1.0 - smargin)), shi2 = (QT)(P.rl2 * (1.0 + smargin));
         constexpr int L = SIM_LIST_LANES;
         for (int a0 = 0; a0 < nh; a0 += SIM_TILE / L) {
             const int a = a0 + tid / L, l = tid % L;
@@ -584,29 +604,32 @@ __global__ void __launch_bounds__(SIM_TILE)
             const TC xi = (TC)x[(size_t)i * STRIDE + 0];
             const TC yi = (TC)x[(size_t)i * STRIDE + 1];
             const TC zi = (TC)x[(size_t)i * STRIDE + 2];
+            const QT xr = (QT)((double)xi - mref[0]), yr = (QT)((double)yi - mref[1]),
+                     zr = (QT)((double)zi - mref[2]);
             const int f0 = (int)(((long long)l * nt) / L), f1 = (int)(((long long)(l + 1) * nt) / L);
             // candidate f: index, image position, column slot; then k_list's test
             auto test = [&](int f, int& sj, unsigned short& slot) {
-                TC px, py, pz;
                 if (staged) {
                     sj = qs[f];
-                    px = qx[f];
-                    py = qy[f];
-                    pz = qz[f];
                     slot = q16[f];
+                    if (sj == i) return false;
+                    const QT ex = xr - qx[f], ey = yr - qy[f], ez = zr - qz[f];
+                    const QT e2 = (ex * ex + ey * ey) + ez * ez;
+                    if (e2 < slo2) return true;
+                    if (e2 > shi2) return false;
                 } else {
                     const int k = cell_of(f);
                     sj = mstart[k] + (f - mpre[k]);
-                    px = (TC)x[(size_t)sj * STRIDE + 0] + msh[k][0];
-                    py = (TC)x[(size_t)sj * STRIDE + 1] + msh[k][1];
-                    pz = (TC)x[(size_t)sj * STRIDE + 2] + msh[k][2];
+                    const TC px = (TC)x[(size_t)sj * STRIDE + 0] + msh[k][0];
+                    const TC py = (TC)x[(size_t)sj * STRIDE + 1] + msh[k][1];
+                    const TC pz = (TC)x[(size_t)sj * STRIDE + 2] + msh[k][2];
                     slot = (unsigned short)(mslot[k] + (f - mpre[k]));
+                    if (sj == i) return false;
+                    const TC ex = xi - px, ey = yi - py, ez = zi - pz;
+                    const TC e2 = (ex * ex + ey * ey) + ez * ez;
+                    if (e2 < lo2) return true;
+                    if (e2 > hi2) return false;
                 }
-                if (sj == i) return false;
-                const TC ex = xi - px, ey = yi - py, ez = zi - pz;
-                const TC e2 = (ex * ex + ey * ey) + ez * ez;
-                if (e2 < lo2) return true;
-                if (e2 > hi2) return false;
                 const TC xj = (TC)x[(size_t)sj * STRIDE + 0];
                 const TC yj = (TC)x[(size_t)sj * STRIDE + 1];
                 const TC zj = (TC)x[(size_t)sj * STRIDE + 2];
