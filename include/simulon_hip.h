@@ -341,6 +341,12 @@ int sim_step_advance(unsigned long long* step_dev, void* stream);
 int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int col1,
                double* chunks, double* thermo, void* stream);
 
+/* sim_reduce that also advances the device step counter (as sim_step_advance)
+ * once the sums are taken: the last call of a step, after the second half has
+ * read the counter, saves the step a launch of its own. */
+int sim_reduce_step(int R, const int* rep_rows, const double* partial, int col0, int col1,
+                    double* chunks, double* thermo, unsigned long long* step_dev, void* stream);
+
 /* Berendsen (barostat.py:36-68) for replicas with barostat = 1: pressure from
  * thermo K + W and the volume, mu, box update in reps; then x, v and
  * pos_last scaled by mu. */

@@ -115,6 +115,7 @@ SIGNATURES = {
                                   _vp, _vp]),
     "sim_step_advance": (_I, [_vp, _vp]),
     "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp]),
+    "sim_reduce_step": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp]),
     "sim_berendsen": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp]),
     "sim_aniso_barostat": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
     "sim_box_deform": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp]),

@@ -192,8 +192,12 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void k_reduce_final(const int* __restrict__ rep_rows, int c0, int c1,
-                               const double* __restrict__ chunks, double* __restrict__ thermo) {
+                               const double* __restrict__ chunks, double* __restrict__ thermo,
+                               unsigned long long* __restrict__ step_dev) {
     const int r = blockIdx.x, c = c0 + (int)threadIdx.x;
+    // the step counter of the noise advances here (after the second half
+    // read it) instead of in a launch of its own
+    if (step_dev != nullptr && r == 0 && threadIdx.x == 0) step_dev[0] += 1ull;
     if (c >= c1) return;
     const int n = rep_rows[r + 1] - rep_rows[r];
     const int nck = max(1, (n + red_len(n) - 1) / red_len(n));
@@ -524,8 +528,8 @@ extern "C" int sim_nl_trigger(int R, void* reps, double* maxd2, int* flag, int* 
     return SIM_OK;
 }
 
-extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int col1,
-                          double* chunks, double* thermo, void* stream) {
+static int reduce_impl(int R, const int* rep_rows, const double* partial, int col0, int col1,
+                          double* chunks, double* thermo, unsigned long long* step_dev, void* stream) {
     if (R <= 0 || !rep_rows || !partial || !chunks || !thermo || col0 < 0 || col1 > SIM_NPART ||
         col0 >= col1)
         return SIM_E_ARG;
@@ -538,9 +542,21 @@ extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int
         hipLaunchKernelGGL(k_reduce_chunks<SIM_NPART>, g, dim3(256), 0, st, rep_rows, partial,
                            col0, col1, chunks);
     hipLaunchKernelGGL(k_reduce_final, dim3(R), dim3(SIM_NPART), 0, st, rep_rows, col0, col1,
-                       chunks, thermo);
+                       chunks, thermo, step_dev);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
+}
+
+extern "C" int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int col1,
+                          double* chunks, double* thermo, void* stream) {
+    return reduce_impl(R, rep_rows, partial, col0, col1, chunks, thermo, nullptr, stream);
+}
+
+extern "C" int sim_reduce_step(int R, const int* rep_rows, const double* partial, int col0,
+                               int col1, double* chunks, double* thermo,
+                               unsigned long long* step_dev, void* stream) {
+    if (!step_dev) return SIM_E_ARG;
+    return reduce_impl(R, rep_rows, partial, col0, col1, chunks, thermo, step_dev, stream);
 }
 
 extern "C" int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double* thermo,

@@ -31,10 +31,18 @@ __device__ __forceinline__ bool rep_active(const int* flag, int r) {
     return flag == nullptr || flag[r] != 0;
 }
 
-__global__ void k_grid(int R, sim_replica_t* reps, const int* flag) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R || !rep_active(flag, r)) return;
-    sim_replica_t& P = reps[r];
+__device__ void grid_of(sim_replica_t& P);
+
+// Cell grid of every rebuilding replica (threads r < R) and the cleared cell
+// counts of their reserved cell ranges (threads c < ncells), in one launch.
+__global__ void k_grid_clear(int R, sim_replica_t* reps, const int* flag, const int* cell_rep,
+                             int ncells, int* cell_count) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < R && rep_active(flag, g)) grid_of(reps[g]);
+    if (g < ncells && rep_active(flag, cell_rep ? cell_rep[g] : 0)) cell_count[g] = 0;
+}
+
+__device__ void grid_of(sim_replica_t& P) {
     const double* h = P.h;
     // perpendicular widths: V / |b x c|, V / |c x a|, V / |a x b|
     double a[3] = {h[0], h[1], h[2]}, b[3] = {h[3], h[4], h[5]}, c[3] = {h[6], h[7], h[8]};
@@ -64,11 +72,6 @@ __global__ void k_grid(int R, sim_replica_t* reps, const int* flag) {
     for (int k = 0; k < 9; ++k) P.hb_inv[k] = P.hinv[k];
 }
 
-__global__ void k_cell_clear(const int* cell_rep, int ncells, const int* flag, int* cell_count) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    if (rep_active(flag, cell_rep ? cell_rep[c] : 0)) cell_count[c] = 0;
-}
 
 template <typename T, int STRIDE>
 __device__ __forceinline__ void load_xyz(const T* x, int i, double& a, double& b, double& c) {
@@ -728,9 +731,8 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms, int* nbr,
                int* nnbr, int n_stride, int max_nbr, int* overflow, hipStream_t st,
                const sim_nl_reorder_t* ro = nullptr) {
-    hipLaunchKernelGGL(k_grid, dim3((R + 63) / 64), dim3(64), 0, st, R, reps, flag);
-    hipLaunchKernelGGL(k_cell_clear, dim3((ncells + 255) / 256), dim3(256), 0, st, cell_rep,
-                       ncells, flag, cell_count);
+    hipLaunchKernelGGL(k_grid_clear, dim3((max(ncells, R) + 255) / 256), dim3(256), 0, st, R,
+                       reps, flag, cell_rep, ncells, cell_count);
     hipLaunchKernelGGL((k_bin<T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, x,
                        flag, cell_count, atom_cell, atom_slot);
     hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag);

@@ -493,32 +493,44 @@ class ReplicaEngine:
              ptr(self.pos4), ptr(self.vel), ptr(self.force), ptr(self.vel_half),
              ptr(self.pos_last), ptr(self.maxd2), stream_ptr())
 
-    def second_half(self, noise=None):
+    def second_half(self, noise=None, advance=True):
+        """O+B; advance=False leaves the noise step counter to the step's last
+        reduction (reduce(..., advance=True))."""
         s = stream_ptr()
         call("sim_integrate_second", ptr(self.reps), ptr(self.tiles), self.ntiles,
              self.types_struct, ptr(self.pos4), ptr(self.vel_half), ptr(self.force), ptr(self.vel),
              ptr(noise), ptr(self.order), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
-        call("sim_step_advance", ptr(self.step_ctr), s)
+        if advance:
+            call("sim_step_advance", ptr(self.step_ctr), s)
 
-    def reduce(self, col0=0, col1=15, ff=None):
+    def reduce(self, col0=0, col1=15, ff=None, advance=False):
         """float64 per-replica sums of the partial columns [col0, col1): force
         columns (< T_KE) from the partials of force field ff (default: the
-        attached one), kinetic columns from the integrator's."""
+        attached one), kinetic columns from the integrator's. advance=True
+        also advances the noise step counter (the step's last reduction)."""
         s = stream_ptr()
         fa, fb = col0, min(col1, _lib.T_KE)
         ka, kb = max(col0, _lib.T_KE), col1
         part, rows = self.force_partials(ff) if fa < fb else (None, None)
+
+        def red(rows_, part_, a, b, last):
+            if advance and last:
+                call("sim_reduce_step", self.R, ptr(rows_), ptr(part_), a, b,
+                     ptr(self.red_chunks), ptr(self.thermo), ptr(self.step_ctr), s)
+            else:
+                call("sim_reduce", self.R, ptr(rows_), ptr(part_), a, b, ptr(self.red_chunks),
+                     ptr(self.thermo), s)
+
         if part is self.partial and rows is self.rep_tiles and ka < kb:
             # force and kinetic columns share the per-tile rows: one pass
-            call("sim_reduce", self.R, ptr(rows), ptr(part), fa, kb, ptr(self.red_chunks),
-                 ptr(self.thermo), s)
+            red(rows, part, fa, kb, True)
             return
         if fa < fb:
-            call("sim_reduce", self.R, ptr(rows), ptr(part), fa, fb, ptr(self.red_chunks),
-                 ptr(self.thermo), s)
+            red(rows, part, fa, fb, not ka < kb)
         if ka < kb:
-            call("sim_reduce", self.R, ptr(self.rep_tiles), ptr(self.partial), ka, kb,
-                 ptr(self.red_chunks), ptr(self.thermo), s)
+            red(self.rep_tiles, self.partial, ka, kb, True)
+        elif advance and not fa < fb:
+            call("sim_step_advance", ptr(self.step_ctr), s)
 
     def force_partials(self, ff=None):
         """(partial rows, per-replica row offsets) the force field writes."""
@@ -580,8 +592,8 @@ class ReplicaEngine:
         self.first_half()
         self.build_list(force=False)
         self.compute_forces(virial=virial or self.has_barostat)
-        self.second_half(noise)
-        self.reduce(0, 15)
+        self.second_half(noise, advance=False)
+        self.reduce(0, 15, advance=True)
         if self.has_barostat:
             self.barostat()
         self.steps_done += 1

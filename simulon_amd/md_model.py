@@ -104,8 +104,8 @@ class BaseModel(nn.Module):
         eng.first_half()
         eng.build_list(force=False)
         eng.compute_forces(virial=True)
-        eng.second_half()
-        eng.reduce(0, 15)
+        eng.second_half(advance=False)
+        eng.reduce(0, 15, advance=True)
         if self.barostat is not None:
             eng.barostat()
         mol._graph_cache = None
