@@ -192,6 +192,8 @@ typedef struct sim_nl_reorder {
     float* vel_half;       /* [N][3] half-step velocities (a rebuild falls between the halves) */
     float* vel_half_tmp;
     int* row_base;         /* [R+1] prefix of the cell-row counts ny*nz, or NULL */
+    int* posq;             /* [N][4] position records rewritten for the reordered
+                              replicas (sim_pos_quantize), or NULL */
 } sim_nl_reorder_t;
 
 int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
@@ -317,6 +319,14 @@ int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
                         sim_types_t types, double* pos4, const float* vel,
                         const float* force, float* vel_half, const double* pos_last,
                         double* maxd2, void* stream);
+
+/* sim_integrate_first that also writes the position records of the new
+ * positions (as sim_pos_quantize) into posq [N][4]: a step whose list build
+ * passes the same posq in sim_nl_reorder_t needs no sim_pos_quantize. */
+int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
+                          sim_types_t types, double* pos4, const float* vel,
+                          const float* force, float* vel_half, const double* pos_last,
+                          double* maxd2, int* posq, void* stream);
 
 /* Second half (O + B): v = vh damp + xi rff (Langevin replicas), then
  * v += 1/2 F/m dt; KE and kinetic-tensor columns of partial.

@@ -78,7 +78,7 @@ class NLReorder(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
                                    "vel_tmp", "force_tmp", "order_tmp", "image_tmp", "cell_tmp",
                                    "nbr16", "stencil_max", "vel_half", "vel_half_tmp",
-                                   "row_base")]
+                                   "row_base", "posq")]
 
 
 # name -> (restype, argtypes)
@@ -111,6 +111,7 @@ SIGNATURES = {
     "sim_eam_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _I,
                            _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sim_integrate_first_q": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,
                                   _vp, _vp]),
     "sim_step_advance": (_I, [_vp, _vp]),

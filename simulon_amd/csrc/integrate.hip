@@ -18,7 +18,8 @@ __global__ void __launch_bounds__(SIM_TILE)
     k_first(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
             sim_types_t ty, double4* __restrict__ pos, const float* __restrict__ vel,
             const float* __restrict__ force, float* __restrict__ vh_out,
-            const double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2) {
+            const double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2,
+            int4* __restrict__ posq) {
     __shared__ double lds[4];
     const SimTile t = tile_of(tiles);
     const sim_replica_t& P = reps[t.rep];
@@ -39,6 +40,7 @@ __global__ void __launch_bounds__(SIM_TILE)
         x.y += (double)v1 * P.dt;
         x.z += (double)v2 * P.dt;
         pos[i] = x;
+        if (posq) posq[i] = quant_record(P, x);   // records for this step's forces
         const double4 l = pos_last[i];
         const double ex = x.x - l.x, ey = x.y - l.y, ez = x.z - l.z;
         d2 = (ex * ex + ey * ey) + ez * ez;
@@ -451,15 +453,7 @@ __global__ void __launch_bounds__(SIM_TILE)
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
     if (i >= P.atom_base + P.natoms) return;
-    const double4 x = pos[i];
-    unsigned u[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        double s = fma(x.z, P.hinv[6 + m], fma(x.y, P.hinv[3 + m], x.x * P.hinv[m]));
-        s -= floor(s);
-        u[m] = (unsigned)(unsigned long long)rint(s * SIM_Q_SCALE);   // 2^32 wraps to 0
-    }
-    q[i] = make_int4((int)u[0], (int)u[1], (int)u[2], (int)x.w);
+    q[i] = quant_record(P, pos[i]);
 }
 
 }  // namespace
@@ -475,10 +469,9 @@ extern "C" int sim_pos_quantize(const void* reps, const int* tiles, int ntiles, 
     return SIM_OK;
 }
 
-extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
-                                   sim_types_t types, double* pos4, const float* vel,
-                                   const float* force, float* vel_half, const double* pos_last,
-                                   double* maxd2, void* stream) {
+static int first_impl(const void* reps, const int* tiles, int ntiles, sim_types_t types,
+                      double* pos4, const float* vel, const float* force, float* vel_half,
+                      const double* pos_last, double* maxd2, int* posq, void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !pos4 || !vel || !force || !vel_half || !pos_last ||
         !maxd2 || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;
@@ -487,9 +480,26 @@ extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntile
                        reinterpret_cast<const SimTile*>(tiles), types,
                        reinterpret_cast<double4*>(pos4), vel, force, vel_half,
                        reinterpret_cast<const double4*>(pos_last),
-                       reinterpret_cast<unsigned long long*>(maxd2));
+                       reinterpret_cast<unsigned long long*>(maxd2), reinterpret_cast<int4*>(posq));
     SIM_CHECK_LAUNCH();
     return SIM_OK;
+}
+
+extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
+                                   sim_types_t types, double* pos4, const float* vel,
+                                   const float* force, float* vel_half, const double* pos_last,
+                                   double* maxd2, void* stream) {
+    return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
+                      nullptr, stream);
+}
+
+extern "C" int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
+                                     sim_types_t types, double* pos4, const float* vel,
+                                     const float* force, float* vel_half, const double* pos_last,
+                                     double* maxd2, int* posq, void* stream) {
+    if (!posq) return SIM_E_ARG;
+    return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
+                      posq, stream);
 }
 
 extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntiles,

@@ -289,13 +289,14 @@ __global__ void k_permute_store(const sim_replica_t* reps, const SimTile* tiles,
                                 const long long* order_t, const int4* image_t, const int* cell_t,
                                 double4* pos, float* vel, float* force, long long* order,
                                 int4* image, int* atom_cell, int* cell_atoms, int* slot_of,
-                                const float* vh_t, float* vh) {
+                                const float* vh_t, float* vh, int4* posq) {
     const SimTile t = tiles[blockIdx.x];
     if (!rep_active(flag, t.rep)) return;
     const sim_replica_t& P = reps[t.rep];
     const int s = t.start + threadIdx.x;
     if (s >= P.atom_base + P.natoms) return;
     pos[s] = pos_t[s];
+    if (posq) posq[s] = quant_record(P, pos_t[s]);   // records of the wrapped positions
     for (int k = 0; k < 3; ++k) {
         vel[(size_t)s * 3 + k] = vel_t[(size_t)s * 3 + k];
         force[(size_t)s * 3 + k] = force_t[(size_t)s * 3 + k];
@@ -761,7 +762,8 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                            ro->force_tmp, ro->order_tmp,
                            reinterpret_cast<const int4*>(ro->image_tmp), ro->cell_tmp, P4,
                            ro->vel, ro->force, ro->order, reinterpret_cast<int4*>(ro->image),
-                           atom_cell, cell_atoms, ro->slot_of, ro->vel_half_tmp, ro->vel_half);
+                           atom_cell, cell_atoms, ro->slot_of, ro->vel_half_tmp, ro->vel_half,
+                           reinterpret_cast<int4*>(ro->posq));
     }
     if (ro != nullptr && tiles != nullptr) {   // positions wrapped and in cell order
         // staged build for replicas with >= 3 cells per dimension (their cell

@@ -26,6 +26,20 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// Fractional fixed-point record of position x in replica P: s = x H^-1
+// (fixed-order products) wrapped into [0, 1), stored as round(s 2^32) mod
+// 2^32 per component; w = atom type.
+__device__ __forceinline__ int4 quant_record(const sim_replica_t& P, const double4& x) {
+    unsigned u[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        double s = fma(x.z, P.hinv[6 + m], fma(x.y, P.hinv[3 + m], x.x * P.hinv[m]));
+        s -= floor(s);
+        u[m] = (unsigned)(unsigned long long)rint(s * SIM_Q_SCALE);   // 2^32 wraps to 0
+    }
+    return make_int4((int)u[0], (int)u[1], (int)u[2], (int)x.w);
+}
+
 // Tile table entry: (replica, first global atom index).
 struct SimTile {
     int rep;

@@ -437,7 +437,9 @@ class ReplicaEngine:
         ro.row_base = self.row_base.data_ptr()
         ro.vel_half, ro.vel_half_tmp = (self.vel_half.data_ptr(),
                                         self._ro_tmp["vel_half"].data_ptr())
+        ro.posq = self.posq.data_ptr()     # a reorder rewrites the records it moves
         self._ro = ro
+        self._posq_fresh = False           # records already current (inside a step)
         self.types_struct = _lib.Types()
         for t, m in enumerate(self.masses_amu):
             self.types_struct.mass[t] = float(internal_mass_f32(m))
@@ -479,16 +481,37 @@ class ReplicaEngine:
 
     def quantize(self):
         """Fractional fixed-point records of the current positions (read by
-        the full-list force kernels)."""
+        the full-list force kernels). Inside a step the first half and the
+        list build have written them already (build_and_forces)."""
+        if self._posq_fresh:
+            return
         call("sim_pos_quantize", ptr(self.reps), ptr(self.tiles), self.ntiles, ptr(self.pos4),
              ptr(self.posq), stream_ptr())
+
+    def build_and_forces(self, virial):
+        """Gated rebuild and forces right after first_half(records=True): the
+        records of the new positions are current (written by the first half,
+        rewritten by a reorder), so the force pass does not quantize again."""
+        self.build_list(force=False)
+        self._posq_fresh = True
+        try:
+            self.compute_forces(virial=virial)
+        finally:
+            self._posq_fresh = False
 
     def compute_forces(self, virial=True):
         if self.force_field is None:
             raise RuntimeError("no force field attached")
         self.force_field.compute(self, virial)
 
-    def first_half(self):
+    def first_half(self, records=False):
+        """B+A; records=True also writes the position records (posq)."""
+        if records:
+            call("sim_integrate_first_q", ptr(self.reps), ptr(self.tiles), self.ntiles,
+                 self.types_struct, ptr(self.pos4), ptr(self.vel), ptr(self.force),
+                 ptr(self.vel_half), ptr(self.pos_last), ptr(self.maxd2), ptr(self.posq),
+                 stream_ptr())
+            return
         call("sim_integrate_first", ptr(self.reps), ptr(self.tiles), self.ntiles, self.types_struct,
              ptr(self.pos4), ptr(self.vel), ptr(self.force), ptr(self.vel_half),
              ptr(self.pos_last), ptr(self.maxd2), stream_ptr())
@@ -589,9 +612,8 @@ class ReplicaEngine:
             virial = True
         if self.loading:
             self.deform()
-        self.first_half()
-        self.build_list(force=False)
-        self.compute_forces(virial=virial or self.has_barostat)
+        self.first_half(records=True)
+        self.build_and_forces(virial or self.has_barostat)
         self.second_half(noise, advance=False)
         self.reduce(0, 15, advance=True)
         if self.has_barostat:

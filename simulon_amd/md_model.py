@@ -101,9 +101,8 @@ class BaseModel(nn.Module):
             mol._trigger_and_build(force=True)
         if eng.loading:
             eng.deform()
-        eng.first_half()
-        eng.build_list(force=False)
-        eng.compute_forces(virial=True)
+        eng.first_half(records=True)
+        eng.build_and_forces(virial=True)
         eng.second_half(advance=False)
         eng.reduce(0, 15, advance=True)
         if self.barostat is not None:
