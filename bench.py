@@ -61,16 +61,28 @@ def timed(fn, steps):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
+def _traffic_record(workload, kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["kernels"].get(f"{workload}/{kernel}")
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, each
-    scaled by what the counter reads for a copy of known size), or None."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            rec = json.load(f)["kernels"].get(f"{workload}/{kernel}")
-    except (OSError, ValueError, KeyError):
-        return None
+    scaled by what the counter reads for a copy of known size), or None. The
+    counter runs are short (5 steps), so their lists are the ones built at the
+    start; traffic_ratio gives the measured bytes over that run's algorithmic
+    bytes, which carries over to a later list state."""
+    rec = _traffic_record(workload, kernel)
     return None if rec is None else int(rec["bytes_per_launch"])
+
+
+def traffic_ratio(workload, kernel):
+    rec = _traffic_record(workload, kernel)
+    return None if rec is None else round(rec["bytes_per_launch"] / rec["algorithmic_bytes"], 4)
 
 
 def step_roofline(natoms, ms_per_step, bytes_per_atom_step):
@@ -146,6 +158,7 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(workload, name),
+            "traffic_ratio": traffic_ratio(workload, name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
             "other_pass_ms": round(t1 if t == t2 else t2, 5)}
 
@@ -175,6 +188,7 @@ def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(workload, name),
+            "traffic_ratio": traffic_ratio(workload, name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2)}
 
 
@@ -443,6 +457,7 @@ def mlp_kernel_roofline(eng):
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic("mlp_w", name),
+            "traffic_ratio": traffic_ratio("mlp_w", name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
             "other_pass_ms": round(t1 if t == t2 else t2, 5),
             "energy_pass_dense_tflops": round(flops / (t1 * 1e-3) / 1e12, 2)}
