@@ -194,14 +194,8 @@ def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
 
 # ---------------------------------------------------------------------------
 def replica_specs(nrep=64):
-    sides = lat.replica_sizes(nrep)
-    temps = np.linspace(100.0, 1500.0, nrep)
-    specs = []
-    for k, (n, T) in enumerate(zip(sides, temps)):
-        pos, H = lat.w_bcc(int(n))
-        specs.append(ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT",
-                                 temperature=float(T), gamma=10.0))
-    return specs
+    return [ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT", temperature=T, gamma=10.0)
+            for pos, H, T in lat.replica_set(nrep)]
 
 
 def force_error_sample(eng, setfl, reps):

@@ -120,6 +120,12 @@ typedef struct sim_replica {
     int axes;             /* anisotropic: bit a set = lattice row a controlled */
     int stochastic;
     int loading;          /* 1: apply deform at the start of every step */
+    int pad2;
+    unsigned long long noise_key; /* this replica's Philox key; the key of its
+                                     Langevin kicks and barostat noise is
+                                     (seed argument) XOR noise_key */
+    double baro_dt;       /* > 0: time step of the barostat update instead of dt
+                             (a standalone barostat call with its own dt) */
 } sim_replica_t;
 
 typedef struct sim_types {

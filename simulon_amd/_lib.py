@@ -41,7 +41,8 @@ class Replica(ctypes.Structure):
         ("max_eta_dot", ctypes.c_double), ("p_tol", ctypes.c_double), ("temp_k", ctypes.c_double),
         ("deform", ctypes.c_double * 3), ("amap", ctypes.c_double * 9),
         ("hb_inv", ctypes.c_double * 9), ("vscale", ctypes.c_float * 3), ("axes", ctypes.c_int),
-        ("stochastic", ctypes.c_int), ("loading", ctypes.c_int),
+        ("stochastic", ctypes.c_int), ("loading", ctypes.c_int), ("pad2", ctypes.c_int),
+        ("noise_key", ctypes.c_uint64), ("baro_dt", ctypes.c_double),
     ]
 
 

@@ -49,11 +49,12 @@ def _load_tensors(eng, kinetic_energy, virial, kinetic_tensor, virial_tensor):
             th[_lib.T_W + k] = 0.0
 
 
-def _set_dt(eng, dt):
-    P = eng.replica_state(0)
-    if P.dt != float(dt):
-        P.dt = float(dt)
-        eng.write_replica_state(0, P)
+def _baro_dt(eng, dt):
+    """Time step of the next barostat update of replica 0 (0: the
+    integrator's). Only the barostat reads this field, so a standalone
+    step(dt) never changes the integrator's time step."""
+    o = _lib.Replica.baro_dt.offset
+    eng.reps[o:o + 8].view(torch.float64).fill_(float(dt))
 
 
 class _DeviceBarostat:
@@ -118,9 +119,10 @@ class BerendsenBarostat(_DeviceBarostat):
     def step(self, dt, kinetic_energy, virial, kinetic_tensor=None, virial_tensor=None) -> float:
         eng = self.molecular.engine
         self._ensure_active()
-        _set_dt(eng, dt)
+        _baro_dt(eng, dt)
         _load_tensors(eng, kinetic_energy, virial, kinetic_tensor, virial_tensor)
         eng.barostat()
+        _baro_dt(eng, 0.0)
         self.molecular._graph_cache = None
         return float(eng.replica_state(0).press)
 
@@ -193,8 +195,9 @@ class AnisotropicNPTBarostat(_DeviceBarostat):
             raise ValueError("AnisotropicNPTBarostat requires kinetic_tensor and virial_tensor")
         eng = self.molecular.engine
         self._ensure_active()
-        _set_dt(eng, dt)
+        _baro_dt(eng, dt)
         _load_tensors(eng, kinetic_energy, virial, kinetic_tensor, virial_tensor)
         eng.barostat()
+        _baro_dt(eng, 0.0)
         self.molecular._graph_cache = None
         return self.pressure_axis_bar.to(torch.float32)

@@ -7,7 +7,9 @@
 // float64 and advanced as x += double(v_half) dt. The first half also measures
 // the displacement since the last list build (io_utils/reader.py:136-149) and
 // folds it into a per-replica maximum; the Langevin kick uses Philox4x32-10
-// with Box-Muller, keyed by (seed, step, global atom index).
+// with Box-Muller: key = seed XOR the replica's noise_key, counter = (atom
+// index within the replica, step), so replicas with different keys draw
+// independent kicks.
 #include "sim_common.h"
 
 namespace {
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(SIM_TILE)
                 g2 = noise[o + 2];
             } else {
                 const uint32_t id = key ? (uint32_t)(key[i] - P.atom_base) : (uint32_t)i;
-                gauss3(seed, step, id, g0, g1, g2);
+                gauss3(seed ^ P.noise_key, step, id, g0, g1, g2);
             }
             const float s = P.rff[ty_i];
             v0 = v0 * P.damp + g0 * s;
@@ -228,7 +230,8 @@ __global__ void k_baro_box(int R, sim_replica_t* reps, const double* thermo) {
     const double tr = (th[SIM_T_K] + th[SIM_T_K + 1] + th[SIM_T_K + 2]) +
                       (th[SIM_T_W] + th[SIM_T_W + 1] + th[SIM_T_W + 2]);
     const double Pbar = tr / P.volume / 3.0 * 1602176.6208;
-    double mu3 = 1.0 + P.kappa * (P.dt / P.tau_p) * (Pbar - P.target_p);
+    const double dt = P.baro_dt > 0.0 ? P.baro_dt : P.dt;
+    double mu3 = 1.0 + P.kappa * (dt / P.tau_p) * (Pbar - P.target_p);
     const double lo = (1.0 - P.mu_max) * (1.0 - P.mu_max) * (1.0 - P.mu_max);
     const double hi = (1.0 + P.mu_max) * (1.0 + P.mu_max) * (1.0 + P.mu_max);
     mu3 = fmin(fmax(mu3, lo), hi);
@@ -341,10 +344,13 @@ __global__ void k_aniso_box(int R, sim_replica_t* reps, const double* thermo,
         S[ia[k]] = v;
         S[ib[k]] = v;
     }
+    const double dt = P.baro_dt > 0.0 ? P.baro_dt : P.dt;
     float gn[3] = {0.f, 0.f, 0.f};
     if (P.stochastic) {
         const unsigned long long step = step_dev ? step_dev[0] : 0ull;
-        gauss3(seed, step, (uint32_t)r, gn[0], gn[1], gn[2], 0xba20u);
+        // a replica with its own key draws the same stream wherever it is packed
+        gauss3(seed ^ P.noise_key, step, P.noise_key ? 0u : (uint32_t)r, gn[0], gn[1], gn[2],
+               0xba20u);
     }
     const double tau2 = fmax(P.tau_p * P.tau_p, 1e-12);
     bool within[3];
@@ -373,13 +379,13 @@ __global__ void k_aniso_box(int R, sim_replica_t* reps, const double* thermo,
     for (int a = 0; a < 3; ++a) {
         if (any_within && within[a]) accel[a] = -hold * P.eta_dot[a];
         const bool ctl = (P.axes >> a) & 1;
-        double e = P.eta_dot[a] + P.dt * accel[a];
+        double e = P.eta_dot[a] + dt * accel[a];
         e = fmin(fmax(e, -P.max_eta_dot), P.max_eta_dot);
         if (!ctl) e = 0.0;
         if (within[a] && fabs(e) < 1e-7) e = 0.0;
         P.eta_dot[a] = e;
-        s[a] = ctl ? exp(P.dt * e) : 1.0;
-        P.vscale[a] = ctl ? (float)exp(-P.dt * e) : 1.0f;
+        s[a] = ctl ? exp(dt * e) : 1.0;
+        P.vscale[a] = ctl ? (float)exp(-dt * e) : 1.0f;
     }
     scale_rows(P, s);
     P.press = (P.p_axis[0] + P.p_axis[1] + P.p_axis[2]) / 3.0;

@@ -77,6 +77,15 @@ def morton_order(pos, H, bits=10):
     return np.argsort(key, kind="stable")
 
 
+def noise_key(seed):
+    """64-bit Philox key of a replica's noise stream (splitmix64 finaliser of
+    its seed, so that nearby seeds give unrelated keys)."""
+    z = (int(seed) + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
 def perpendicular_widths(H):
     H = np.asarray(H, np.float64)
     V = abs(np.linalg.det(H))
@@ -101,7 +110,7 @@ class ReplicaSpec:
     tau_p: float = 1.0                    # ps
     kappa: float = 4.57e-5                # 1/bar (barostat.py:27)
     mu_max: float = 0.05                  # barostat.py:28
-    seed: int = 0
+    seed: int = None                      # Langevin noise stream; None: engine seed + replica index
     barostat: str = "berendsen"           # NPT controller: berendsen | anisotropic
 
 
@@ -132,7 +141,7 @@ class ReplicaEngine:
         self.cutoff = float(cutoff)
         self.skin = float(skin)
         self.precise_list = bool(precise_list)
-        self.seed = int(seed)
+        self._seed = int(seed)
         self.sort = bool(sort)
         _lib.load()
         self._layout()
@@ -337,11 +346,45 @@ class ReplicaEngine:
                 P.rff[t] = float(rff)
         P.damp = float(damp)
         P.rcf = float(np.float32(self.cutoff))
+        P.noise_key = self.replica_noise_key(r)
         P.atom_base = int(self.atom_base[r])
         P.natoms = self.natoms[r]
         P.cell_base = int(self.cell_base[r])
         P.cell_cap = int(self.cell_cap[r])
         return P
+
+    def replica_seed(self, r):
+        """Seed of replica r's Langevin noise: its spec's seed, else the
+        engine seed + r (a replica packed with others draws the same kicks as
+        a single-replica engine of that seed)."""
+        s = self.specs[r].seed
+        return self._seed + r if s is None else int(s)
+
+    def replica_noise_key(self, r):
+        return noise_key(self.replica_seed(r))
+
+    @property
+    def seed(self):
+        return self._seed
+
+    @seed.setter
+    def seed(self, value):
+        """A new engine seed re-keys the replicas that have no seed of their own."""
+        self._seed = int(value)
+        if getattr(self, "reps", None) is None:
+            return
+        o = _lib.Replica.noise_key.offset
+        sz = ctypes.sizeof(_lib.Replica)
+        keys = np.array([self.replica_noise_key(r) for r in range(self.R)], np.uint64)
+        raw = torch.from_numpy(keys.view(np.uint8).reshape(self.R, 8).copy())
+        self.reps.view(self.R, sz)[:, o:o + 8].copy_(raw.to(self.device))
+        self._graph = None
+
+    def noise_keys(self):
+        """[R] int64 view of the replicas' Philox keys (device)."""
+        o = _lib.Replica.noise_key.offset
+        sz = ctypes.sizeof(_lib.Replica)
+        return self.reps.view(self.R, sz)[:, o:o + 8].contiguous().view(torch.int64).view(self.R)
 
     def _alloc(self, max_nbr, velocity_seed):
         dev, N, R = self.device, self.N, self.R
@@ -349,7 +392,7 @@ class ReplicaEngine:
         pos = np.zeros((N, 4), np.float64)
         vel = np.zeros((N, 3), np.float32)
         order = np.arange(N, dtype=np.int64)
-        rng = np.random.default_rng(self.seed if velocity_seed is None else velocity_seed)
+        rng = np.random.default_rng(self._seed if velocity_seed is None else velocity_seed)
         for r, s in enumerate(self.specs):
             b, e = self.atom_base[r], self.atom_base[r + 1]
             x = np.asarray(s.positions, np.float64)
@@ -522,7 +565,7 @@ class ReplicaEngine:
         s = stream_ptr()
         call("sim_integrate_second", ptr(self.reps), ptr(self.tiles), self.ntiles,
              self.types_struct, ptr(self.pos4), ptr(self.vel_half), ptr(self.force), ptr(self.vel),
-             ptr(noise), ptr(self.order), self.seed, 0, ptr(self.step_ctr), ptr(self.partial), s)
+             ptr(noise), ptr(self.order), 0, 0, ptr(self.step_ctr), ptr(self.partial), s)
         if advance:
             call("sim_step_advance", ptr(self.step_ctr), s)
 
@@ -572,7 +615,7 @@ class ReplicaEngine:
         if "anisotropic" in kinds:
             call("sim_aniso_barostat", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
                  ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last),
-                 self.seed, ptr(self.step_ctr), stream_ptr())
+                 0, ptr(self.step_ctr), stream_ptr())
 
     def deform(self):
         """Scale the lattice rows of the replicas with loading = 1 by their

@@ -88,6 +88,26 @@ class EAMForceField:
             p.inv_drho[k] = float(np.float32(t.inv_drho[a]))
         self.params = p
 
+    def embedding_energy(self, eng):
+        """[R] float64 sums of F_t(rho_i) over each replica's atoms from the
+        densities of the last density pass, with the kernels' float32 clamp
+        and interpolation (core/force/eam_force_cu.py:448-459, 577)."""
+        rho = eng.scratch("eam_rho", torch.float32)
+        t = eng.pos4[:, 3].long()
+        p = self.params
+        lo = torch.tensor([p.rho_lo[k] for k in range(p.ntypes)], dtype=torch.float32,
+                          device=eng.device)[t]
+        hi = torch.tensor([p.rho_hi[k] for k in range(p.ntypes)], dtype=torch.float32,
+                          device=eng.device)[t]
+        inv = torch.tensor([p.inv_drho[k] for k in range(p.ntypes)], dtype=torch.float32,
+                           device=eng.device)[t]
+        xf = (torch.minimum(torch.maximum(rho, lo), hi) - lo) * inv
+        idx = xf.to(torch.int64)
+        fr = xf - idx.to(torch.float32)
+        e = self.dev["emb"][t, idx]                       # [N, 2]: F(k), F(k+1)
+        Fi = (e[:, 0] + fr * (e[:, 1] - e[:, 0])).double()
+        return torch.stack([Fi[eng.replica_slice(r)].sum() for r in range(eng.R)])
+
     def compute(self, eng, virial=True, force=None, partial=None):
         """force/partial: optional output arrays in place of the engine's
         (float [N][3] and double [ntiles][16]; the global-list kernels run)."""

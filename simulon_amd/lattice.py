@@ -70,6 +70,14 @@ def replica_sizes(nrep=64, lo=1000, hi=64000, seed=3):
     return side.astype(int)
 
 
+def replica_set(nrep=64, t_lo=100.0, t_hi=1500.0):
+    """The temperature-scan workload: nrep W bcc cells (a = 3.165 A) of the
+    sizes replica_sizes(nrep) draws, with Langevin temperatures evenly spaced
+    over [t_lo, t_hi]. Returns [(positions, box, temperature)]."""
+    temps = np.linspace(t_lo, t_hi, nrep)
+    return [(*w_bcc(int(n)), float(T)) for n, T in zip(replica_sizes(nrep), temps)]
+
+
 def taper(r, r0=5.0, r1=5.5):
     """Cubic taper 1 - 3t^2 + 2t^3 from 1 at r0 to 0 at r1 (C1 at both ends)."""
     t = np.clip((np.asarray(r) - r0) / (r1 - r0), 0.0, 1.0)

@@ -177,15 +177,16 @@ class MDSimulator:
         """BaseModel.run's start: the first force evaluation and a pending rebuild."""
         model, eng, mol = self.model, self.model.engine, self.model.molecular
         if model._first_call:
-            eng.compute_forces(virial=True)
-            eng.reduce(0, 8)
+            model._initial_forces()
             model._first_call = False
         if mol.needs_update:
             mol._trigger_and_build(force=True)
 
     def _advance(self, k):
-        eng = self.model.engine
-        eng.run(k, graph=True, virial=True, check=False)
+        if getattr(self.model, "extra", None):
+            self.model.run(k)          # host backbones: eager steps
+        else:
+            self.model.engine.run(k, graph=True, virial=True, check=False)
         self.model.molecular._graph_cache = None
 
     def _scalars(self):

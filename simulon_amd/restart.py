@@ -4,8 +4,8 @@ save_checkpoint(model, step, path) writes the reference's keys
 (restart.py:32-74): step, coordinates, velocities, box_H, rng_cpu, rng_cuda,
 vel_half, force_cache, energy_cache, virial_cache, all in the caller's atom
 order. Under the extra key "simulon_amd" it adds what the device step needs to
-continue the same trajectory: the float64 coordinates, the Langevin noise seed
-and device step counter (the noise of an atom is keyed by its caller index and
+continue the same trajectory: the float64 coordinates, the Langevin noise seed,
+the replica's Philox key and the device step counter (the noise of an atom is keyed by its caller index and
 that counter, so it continues unchanged), the virial tensor, the last
 pressure, the anisotropic barostat's strain rates and the reduced thermo row.
 
@@ -45,6 +45,7 @@ def save_checkpoint(model, step: int, path: str | Path) -> None:
         _KEY: {
             "coordinates_f64": eng.to_caller(eng.positions(torch.float64, unwrap=True)).cpu(),
             "seed": int(eng.seed),
+            "noise_key": eng.noise_keys().cpu().clone(),
             "step_counter": eng.step_ctr.detach().cpu().clone(),
             "thermo": th.detach().cpu().clone(),
             "pressure": eng.reps[o:o + 8].view(torch.float64).detach().cpu().clone(),
@@ -66,6 +67,13 @@ def load_checkpoint(model, path: str | Path) -> int:
     ext = ckpt.get(_KEY, {})
     if "box_H" in ckpt:
         mol.box.H = ckpt["box_H"].to(device=dev, dtype=torch.float64)
+    elif "box_length" in ckpt:
+        # scalar-box checkpoints (io_utils/restart.py:59-60): a cubic box
+        bl = ckpt["box_length"]
+        L = float(bl.reshape(-1)[0].double()) if torch.is_tensor(bl) else float(bl)
+        mol.box.H = torch.eye(3, dtype=torch.float64, device=dev) * L
+    else:
+        raise KeyError("checkpoint holds neither 'box_H' nor 'box_length'")
     x = ext.get("coordinates_f64", ckpt["coordinates"])
     mol.coordinates = x.to(dev)
     mol.atom_velocities = ckpt["velocities"].to(dev)
@@ -82,6 +90,9 @@ def load_checkpoint(model, path: str | Path) -> int:
             eng.thermo[0, _lib.T_VIRIAL] = float(ckpt["virial_cache"])
     if "seed" in ext:
         eng.seed = int(ext["seed"])
+    if "noise_key" in ext:
+        o = _lib.Replica.noise_key.offset
+        eng.reps[o:o + 8].view(torch.int64).copy_(ext["noise_key"][:1].to(dev))
     if "step_counter" in ext:
         eng.step_ctr.copy_(ext["step_counter"].to(dev))
     if "pressure" in ext:

@@ -2,7 +2,7 @@
 reference's own pure-Python modules (torch CPU) from /root/reference.
 
 Run in the build container only (the reference is not present elsewhere):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [part ...]   (default: every part)
 
 Modules driven (all importable with torch/numpy/scipy only):
   core/box.py                         Box.minimum_image (FP32 and FP64)
@@ -12,6 +12,7 @@ Modules driven (all importable with torch/numpy/scipy only):
   io_utils/eam_parser.py              EAMParser
   core/integrator/integrator.py       VerletIntegrator first_half / second_half
   core/barostat.py                    BerendsenBarostat.step
+  io_utils/restart.py                 save_checkpoint (restart_ref_*.pt fixtures)
 io_utils/reader.py cannot be imported here (pymatgen, torch_geometric are not
 installed), so the few molecular-object behaviours the modules above call are
 provided by a small stub below: the i<j re-filter and the edge distances
@@ -41,6 +42,7 @@ from core.force.eam_force_cu import EAMForceCUDAExt  # noqa: E402
 from io_utils.eam_parser import EAMParser  # noqa: E402
 from core.integrator.integrator import VerletIntegrator  # noqa: E402
 from core.barostat import BerendsenBarostat  # noqa: E402
+from io_utils.restart import save_checkpoint  # noqa: E402
 
 from simulon_amd import lattice as lat  # noqa: E402  (seeded inputs only)
 
@@ -198,28 +200,73 @@ def golden_barostat(out):
                baro_v1=mol.atom_velocities.numpy())
 
 
-def main():
+def golden_restart():
+    """Checkpoints written by the reference's save_checkpoint
+    (io_utils/restart.py:32-78) for a model in the middle of an NVE argon run
+    (1000 atoms, the 10^3 simple-cubic cell): coordinates, velocities, the
+    integrator's vel_half and the force/energy/virial caches of the
+    reference's LennardJonesForce at those coordinates. restart_ref_boxH.pt
+    holds the box as 'box_H' (a molecular with a Box); restart_ref_boxlen.pt
+    as the scalar 'box_length' (a molecular without one, restart.py:59-60)."""
+    rng = np.random.default_rng(61)
+    pos, H = lat.lj_argon_nvt(seed=31)
+    box = Box(torch.tensor(H), device=CPU)
+    p = torch.tensor(pos, dtype=torch.float32)
+    ei, d = edge_list(p, 8.5, 1.0, box)
+    E = ei.shape[1]
+    mol = Mol(graph_data=types.SimpleNamespace(pos=p, edge_index=ei, edge_attr=d),
+              pair_params=[torch.full((E,), 0.0104), torch.full((E,), 3.405)],
+              cutoff=torch.tensor(8.5), is_fs=False, is_switch=False, switch_ratio=0.9, box=box,
+              coordinates=p, device=CPU)
+    r = LennardJonesForce(mol)()
+    m = 39.948 * 1.036427e-4
+    v = torch.tensor(rng.standard_normal((len(pos), 3)) * np.sqrt(8.617333262e-5 * 94.4 / m),
+                     dtype=torch.float32)
+    v = v - v.mean(0)
+    vh = v + 0.5 * r["forces"] / np.float32(39.948 * 10.0 / (6.02214076e23 * 1.60218e-19)) * 0.002
+    for name, with_box in [("boxH", True), ("boxlen", False)]:
+        attrs = dict(coordinates=p.clone(), atom_velocities=v.clone(), device=CPU)
+        if with_box:
+            attrs["box"] = box
+        else:
+            attrs["box_length"] = float(H[0, 0])
+        model = types.SimpleNamespace(
+            molecular=types.SimpleNamespace(**attrs),
+            Integrator=types.SimpleNamespace(vel_half=vh.clone()),
+            force_cache=r["forces"].detach().clone(), energy_cache=r["energy"].detach().clone(),
+            virial_cache=r["virial"].detach().clone())
+        path = os.path.join(HERE, f"restart_ref_{name}.pt")
+        save_checkpoint(model, 41, path)
+        print("wrote", path, flush=True)
+
+
+def main(which=None):
     torch.set_num_threads(1)
     tmp = tempfile.mkdtemp()
     parts = [("minimage", golden_min_image), ("pairs", golden_pairs), ("lj", golden_lj),
              ("integrator", golden_integrator), ("barostat", golden_barostat)]
     for name, fn in parts:
+        if which and name not in which:
+            continue
         out = {}
         fn(out)
         np.savez_compressed(os.path.join(HERE, f"golden_{name}.npz"), **out)
         print("wrote", name, len(out), flush=True)
-    out = {}
-    golden_eam(out, tmp)
-    np.savez_compressed(os.path.join(HERE, "golden_eam.npz"), **out)
-    print("wrote eam", len(out))
-    # the reference's W-Re potential file, used by its own regression tests
-    src = os.path.join(REF, "run_data", "W", "WRe_YC2.eam.fs")
-    dst = os.path.join(HERE, "data", "WRe_YC2.eam.fs.gz")
-    os.makedirs(os.path.dirname(dst), exist_ok=True)
-    with open(src, "rb") as fi, gzip.open(dst, "wb", compresslevel=9) as fo:
-        shutil.copyfileobj(fi, fo)
+    if not which or "eam" in which:
+        out = {}
+        golden_eam(out, tmp)
+        np.savez_compressed(os.path.join(HERE, "golden_eam.npz"), **out)
+        print("wrote eam", len(out))
+        # the reference's W-Re potential file, used by its own regression tests
+        src = os.path.join(REF, "run_data", "W", "WRe_YC2.eam.fs")
+        dst = os.path.join(HERE, "data", "WRe_YC2.eam.fs.gz")
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        with open(src, "rb") as fi, gzip.open(dst, "wb", compresslevel=9) as fo:
+            shutil.copyfileobj(fi, fo)
+    if not which or "restart" in which:
+        golden_restart()
     shutil.rmtree(tmp)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -161,6 +161,11 @@ def test_eam_backbone_matches_reference(case, syn_file, wre_file):
     print(f"eam {case} err vs reference fp32 {err:.2e}", flush=True)
     assert err < 2e-5
     assert float(out["energy"]) == pytest.approx(float(g[f"eam_{case}_energy"]), rel=1e-6)
+    # the energy split of eam_force_cu.py:577-578
+    for k in ("embedding_energy", "pair_potential"):
+        assert float(out[k]) == pytest.approx(float(g[f"eam_{case}_{k}"]), rel=1e-5, abs=1e-4)
+    assert float(out["embedding_energy"] + out["pair_potential"]) == \
+        pytest.approx(float(out["energy"]), rel=1e-12)
 
 
 def test_base_model_nvt_thermalises():
@@ -216,3 +221,123 @@ def test_base_model_npt_barostat_moves_box():
     # box, coordinates and the device record stay consistent
     H1 = mol.box.H.cpu().numpy()
     np.testing.assert_allclose(H1 / H1[0, 0], H / H[0, 0], rtol=1e-12)
+
+
+def test_maxwell_boltzmann_initialisation():
+    """reader.py:319-339: velocities ~ N(0, kB T / (m 1.036427e-4)) per
+    component with the centre-of-mass velocity removed, both for the
+    engine's own initialisation (ReplicaSpec without velocities, at t_init)
+    and for AtomFileReader.set_maxwell_boltzmann_velocity (called by the NVT
+    integrator with T_init, integrator.py:82)."""
+    from simulon_amd.engine import KB, MB_MASS_CONV, ReplicaEngine, ReplicaSpec
+    from simulon_amd.integrator import VerletIntegrator
+    from simulon_amd.reader import AtomFileReader
+    pos, H = lat.w_bcc(20)
+    n = len(pos)
+
+    def check(v, mass_amu, T, tag):
+        v = v.astype(np.float64)
+        m = mass_amu * MB_MASS_CONV
+        sig = np.sqrt(KB * T / m)
+        com = np.abs(v.sum(0)) / (n * sig)
+        var = (v * v).mean(0) / sig ** 2
+        print(f"{tag}: |COM| / (N sigma) {com.max():.2e}, variance ratio {var}", flush=True)
+        assert com.max() < 1e-6
+        # 3 sigma of the sample variance of n draws is 3 sqrt(2/n) = 3.4% at n = 16000
+        assert np.all(np.abs(var - 1.0) < 0.035)
+        # and the normal shape: fourth moment 3 sigma^4
+        k4 = (v ** 4).mean(0) / sig ** 4 / var ** 2
+        assert np.all(np.abs(k4 - 3.0) < 0.2)
+
+    eng = ReplicaEngine([ReplicaSpec(positions=pos, box=H, ensemble="NVT", temperature=900.0,
+                                     t_init=300.0)], [lat.W_MASS], cutoff=5.5, skin=1.0, seed=5)
+    check(eng.to_caller(eng.vel).cpu().numpy(), lat.W_MASS, 300.0, "engine init")
+    torch.manual_seed(3)
+    mol = AtomFileReader(atom_types=["W"] * n, positions=pos, box_length=float(H[0, 0]), cutoff=5.5,
+                         device="cuda", skin_thickness=1.0)
+    VerletIntegrator(mol, dt=0.001, ensemble="NVT", temperature=(450.0, 300.0), gamma=10.0)
+    check(mol.atom_velocities.cpu().numpy(), lat.W_MASS, 450.0, "reader init")
+
+
+class _Wall(torch.nn.Module):
+    """A host backbone: a repulsive sphere U = K/3 (R - d)^3 for d < R around
+    c, F_i = K (R - d)^2 (x_i - c)/d (the form of the reference's
+    core/mechanics/indenter.py), evaluated in torch on the caller's
+    coordinates."""
+
+    def __init__(self, mol, center, radius=4.0, k=0.05):
+        super().__init__()
+        self.mol, self.c, self.R, self.K = mol, torch.tensor(center, dtype=torch.float64), radius, k
+
+    @staticmethod
+    def terms(x, c, R, K):
+        d_vec = x - c
+        d = np.linalg.norm(d_vec, axis=1)
+        p = np.maximum(R - d, 0.0)
+        f = (K * p ** 2 / np.maximum(d, 1e-12))[:, None] * d_vec
+        return K / 3.0 * float((p ** 3).sum()), f, np.einsum("ni,nj->ij", f, x)
+
+    def forward(self):
+        x = self.mol.coordinates.double()
+        c = self.c.to(x.device)
+        d_vec = x - c
+        d = d_vec.norm(dim=1).clamp_min(1e-12)
+        p = (self.R - d).clamp_min(0.0)
+        f = (self.K * p ** 2 / d)[:, None] * d_vec
+        W = torch.einsum("ni,nj->ij", f, x)
+        return {"energy": self.K / 3.0 * (p ** 3).sum(), "forces": f.float(),
+                "virial": torch.trace(W), "virial_tensor": W}
+
+
+def test_base_model_sums_device_and_host_backbones():
+    """SumBackboneInterface([device LJ, host wall]) under BaseModel
+    (md_model.py:19-52): the step integrates the summed forces, and the
+    outputs carry the summed energy, forces and virial. 100 NVE steps
+    against the float64 oracle trajectory of LJ + wall."""
+    from oracle import cell as ocell
+    from simulon_amd.force import LennardJonesForce
+    from simulon_amd.integrator import VerletIntegrator
+    from simulon_amd.md_model import BaseModel, SumBackboneInterface
+    from simulon_amd.reader import AtomFileReader
+    torch.manual_seed(1)
+    pos, H = lat.lj_argon_nvt(seed=5)
+    mol = AtomFileReader(atom_types=["Ar"] * len(pos), positions=pos, box_length=36.4, cutoff=8.5,
+                         device="cuda", parameter={"[0 0]": {"epsilon": 0.0104, "sigma": 3.405}},
+                         skin_thickness=1.0)
+    integ = VerletIntegrator(mol, dt=0.002, ensemble="NVE")
+    mol.atom_velocities = torch.zeros((len(pos), 3))
+    c = [20.02, 20.02, 20.02]        # the centre of a lattice cube: 8 atoms at 3.15 A
+    wall = _Wall(mol, c)
+    model = BaseModel(SumBackboneInterface([LennardJonesForce(mol), wall], mol), integ, mol)
+    assert model.extra == [wall]
+    eng = model.engine
+    x = eng.to_caller(eng.positions(torch.float64, unwrap=True)).cpu().numpy()
+    v = np.zeros_like(x)
+    m = float(eng.types_struct.mass[0])
+
+    def total(x):
+        o = ocell.lj(x, H, None, [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
+        ew, fw, ww = _Wall.terms(x, np.array(c), wall.R, wall.K)
+        return o["energy"] + ew, o["forces"] + fw, o["virial_tensor"] + ww
+
+    e, f, W = total(x)
+    worst_f = worst_e = 0.0
+    for s in range(100):
+        out = model()
+        vh = v + 0.5 * f / m * 0.002
+        x = x + vh * 0.002
+        e, f, W = total(x)
+        v = vh + 0.5 * f / m * 0.002
+        if s in (0, 99):
+            fe = np.max(np.linalg.norm(out["forces"].cpu().numpy() - f, axis=1)) / \
+                np.max(np.linalg.norm(f, axis=1))
+            worst_f = max(worst_f, fe)
+            # float32 pair terms of 1000 atoms, and a float32-coordinate wall
+            assert np.max(np.abs(out["virial_tensor"].cpu().numpy() - W)) <= 2e-5 * np.max(np.abs(W))
+        worst_e = max(worst_e, abs(float(out["energy"]) - e))
+    ew0 = _Wall.terms(x, np.array(c), wall.R, wall.K)[0]
+    print(f"LJ + wall: force err {worst_f:.2e}, max |dE| {worst_e:.2e} eV, wall energy {ew0:.3e} eV",
+          flush=True)
+    assert ew0 > 1e-3                       # the wall acts on the trajectory
+    assert worst_f <= 1e-5
+    assert worst_e <= 1e-3

@@ -188,25 +188,63 @@ def test_integrator_injected_noise():
 
 
 def test_batched_equals_single():
+    """Three NVT replicas packed in one engine advance exactly as three
+    single-replica engines with the same positions, caller-order velocities,
+    per-replica noise seeds and pair-kernel family: 30 graph-replayed steps,
+    positions and velocities bit for bit."""
     sizes = [3, 4, 5]
-    specs, singles = [], []
+    specs = []
     for k, n in enumerate(sizes):
         pos, H = lat.w_bcc(2 * n, seed=10 + k, amp=0.05)
         specs.append(ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT",
                                  temperature=300 + 100 * k, gamma=5.0))
-    rng_v = [None] * len(specs)
-    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.0, skin=1.0)
+    eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.0, skin=1.0, seed=77)
     eng.force_field = LJForceField([[0.1]], [[2.5]], 5.0)
     eng.setup()
-    f_all = eng.force.cpu().numpy()
+    v_caller = eng.to_caller(eng.vel).cpu().numpy()
+    singles = []
     for k, s in enumerate(specs):
-        s2 = ReplicaSpec(positions=s.positions, box=s.box, velocities=eng.vel[eng.replica_slice(k)].cpu().numpy(),
-                         dt=0.001)
-        e1 = ReplicaEngine([s2], [lat.W_MASS], cutoff=5.0, skin=1.0)
+        s2 = ReplicaSpec(positions=s.positions, box=s.box,
+                         velocities=v_caller[eng.replica_slice(k)], dt=0.001, ensemble="NVT",
+                         temperature=s.temperature, gamma=s.gamma, seed=eng.replica_seed(k))
+        e1 = ReplicaEngine([s2], [lat.W_MASS], cutoff=5.0, skin=1.0, family=eng.family)
         e1.force_field = LJForceField([[0.1]], [[2.5]], 5.0)
         e1.setup()
-        np.testing.assert_array_equal(e1.force.cpu().numpy(), f_all[eng.replica_slice(k)])
-    del rng_v
+        np.testing.assert_array_equal(e1.force.cpu().numpy(),
+                                      eng.force.cpu().numpy()[eng.replica_slice(k)])
+        singles.append(e1)
+    eng.run(30, graph=True)
+    for e1 in singles:
+        e1.run(30, graph=True)
+    x_all = eng.to_caller(eng.positions(unwrap=True)).cpu().numpy()
+    v_all = eng.to_caller(eng.vel).cpu().numpy()
+    for k, e1 in enumerate(singles):
+        sl = eng.replica_slice(k)
+        np.testing.assert_array_equal(e1.to_caller(e1.positions(unwrap=True)).cpu().numpy(),
+                                      x_all[sl])
+        np.testing.assert_array_equal(e1.to_caller(e1.vel).cpu().numpy(), v_all[sl])
+    assert [int(x) for x in eng.nbuild.cpu()] == [int(e.nbuild.cpu()[0]) for e in singles]
+
+
+def test_replica_noise_streams_independent():
+    """Two identical NVT replicas in one engine (default seeds) draw
+    independent Langevin kicks: after one step v_a - v_b = rff (xi_a - xi_b),
+    whose spread is sqrt(2) rff for independent draws and 0 for shared ones."""
+    pos, H = lat.w_bcc(10, seed=4, amp=0.05)
+    spec = dict(positions=pos, box=H, velocities=np.zeros((len(pos), 3)), dt=0.001,
+                ensemble="NVT", temperature=600.0, gamma=10.0)
+    eng = ReplicaEngine([ReplicaSpec(**spec), ReplicaSpec(**spec)], [lat.W_MASS], cutoff=5.0,
+                        skin=1.0, seed=3)
+    eng.force_field = LJForceField([[0.1]], [[2.5]], 5.0)
+    eng.setup()
+    eng.step()
+    v = eng.to_caller(eng.vel).cpu().numpy().astype(np.float64)
+    n = len(pos)
+    rff = float(eng.replica_state(0).rff[0])
+    d = (v[:n] - v[n:]) / rff
+    ratio = float(d.std()) / np.sqrt(2.0)
+    print(f"kick difference spread / sqrt(2): {ratio:.4f}", flush=True)
+    assert 0.96 < ratio < 1.04
 
 
 def test_nve_trajectory_vs_fp64_oracle():

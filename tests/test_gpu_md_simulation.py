@@ -160,3 +160,54 @@ def test_simulator_segments_match_stepwise_loop(tmp_path):
     # energy is not conserved here and is not what this test checks)
     tot = np.array([float(r.split(",")[3]) for r in rows])
     print(f"NVE 2100 steps: total energy {tot[0]:.4f} -> {tot[-1]:.4f} eV", flush=True)
+
+
+def test_simulator_energies_follow_fp64_oracle(tmp_path):
+    """energies.csv and traj.xyz of an NVE run through MDSimulator against the
+    float64 oracle's velocity-Verlet trajectory (oracle/cell_md.c forces,
+    core/force/lennard_jones_force.py) started from the same state: every
+    row's potential, kinetic and total energy and every frame's coordinates."""
+    from oracle import cell as ocell
+    from simulon_amd.md_simulation import MDSimulator
+    print("[case] md_simulation NVE 200 steps vs fp64 oracle", flush=True)
+    nsteps, every = 200, 10
+    torch.manual_seed(0)
+    mol, model = _model("NVE")
+    eng = model.engine
+    x = eng.to_caller(eng.positions(torch.float64, unwrap=True)).cpu().numpy()
+    v = mol.atom_velocities.cpu().numpy().astype(np.float64)
+    H = eng.box(0)
+    m = float(eng.types_struct.mass[0])
+    dt = 0.002
+
+    def lj(x):
+        return ocell.lj(x, H, None, [[lat.AR_EPS]], [[lat.AR_SIG]], 8.5, 1.0)
+
+    out = lj(x)
+    ref = {}
+    for s in range(1, nsteps + 1):
+        vh = v + 0.5 * out["forces"] / m * dt
+        x = x + vh * dt
+        out = lj(x)
+        v = vh + 0.5 * out["forces"] / m * dt
+        if s % every == 0:
+            ke = 0.5 * m * float((v * v).sum())
+            ref[s] = (out["energy"], ke, x.copy())
+    sim = MDSimulator(model, nsteps, print_interval=0, output_dir=str(tmp_path),
+                      traj_interval=50, write_energies=True, energies_interval=every)
+    sim.run(enable_minimize_energy=False)
+    rows = (tmp_path / "energies.csv").read_text().splitlines()[1:]
+    assert [int(r.split(",")[0]) for r in rows] == list(range(every, nsteps + 1, every))
+    worst = 0.0
+    for r in rows:
+        s, epot, ke, tot, T = (float(v) for v in r.split(","))
+        e_or, ke_or, _ = ref[int(s)]
+        worst = max(worst, abs(epot - e_or), abs(ke - ke_or), abs(tot - (e_or + ke_or)))
+        T_or = (2.0 / 3.0) * ke_or / (mol.atom_count * 8.617333262e-5)
+        assert abs(T - T_or) <= 1e-3
+    frames = _read_xyz(tmp_path / "traj.xyz")
+    dx = max(float(np.abs(xf - ref[s][2]).max()) for (_, _, xf), s in zip(frames, (50, 100, 150, 200)))
+    print(f"driver vs fp64 oracle: max |dE| {worst:.2e} eV, max |dx| {dx:.2e} A", flush=True)
+    assert len(frames) == 4
+    assert worst <= 1e-3
+    assert dx <= 1e-4
