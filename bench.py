@@ -121,10 +121,10 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     ff = eng.force_field
     nbar = mean_full_list(eng)
 
-    rho = eng.scratch("eam_rho", torch.float32)
-    fp = eng.scratch("eam_fp", torch.float32)
     fam = eng.family if eng.use_tiled else "global"
     if fam != "global":
+        rho = eng.scratch("eam_rho", torch.float32)
+        fp = eng.scratch("eam_fp", torch.float32)
         common = (*(eng.row_args() if fam == "rows" else eng.cell_args()), ff.params)
 
         def p1():
@@ -138,9 +138,9 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
 
-        def p1():
+        def p1():      # as the step launches it: records with F', no rho / F' arrays
             call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
-                 ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(rho), ptr(fp),
+                 ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, None, None,
                  ptr(rec), ptr(eng.partial), stream_ptr())
 
         def p2():

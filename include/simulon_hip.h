@@ -257,11 +257,13 @@ int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
                  sim_lj_params_t lj, float* force, double* partial, int virial,
                  void* stream);
 
-/* EAM (core/force/eam_force_cu.py:461-579). Pass 1: rho_i, F(rho_i),
- * F'(rho_i) (written to fp[N]) and the energy column; rec (optional, int[N][4],
- * not aliasing posq) receives the records with F'(rho_i) in place of the
- * type, for pass 2. Pass 2: forces and virial columns from rec (positions and
- * F' of the neighbours) and posq (atom types, read when ntypes > 1). */
+/* EAM (core/force/eam_force_cu.py:461-579). Pass 1: rho_i (written to rho[N]
+ * when rho is not NULL), F(rho_i), F'(rho_i) (written to fp[N] when fp is not
+ * NULL) and the energy column; rec (int[N][4], not aliasing posq) receives the
+ * records with F'(rho_i) in place of the type, for pass 2. At least one of fp
+ * and rec must be given. Pass 2: forces and virial columns from rec
+ * (positions and F' of the neighbours) and posq (atom types, read when
+ * ntypes > 1). */
 int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
                     const int* posq, const int* nbr, const int* nnbr, int n_stride,
                     sim_eam_params_t eam, float* rho, float* fp, int* rec, double* partial,

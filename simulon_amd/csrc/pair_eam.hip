@@ -222,8 +222,9 @@ __global__ void __launch_bounds__(SIM_TILE * TPW)
             const float2 F = reinterpret_cast<const float2*>(E.emb)[(size_t)ti * E.n_rho + idx];
             const float2 dF = reinterpret_cast<const float2*>(E.demb)[(size_t)ti * E.n_rho + idx];
             const float fpi = lerp2(dF.x, dF.y, fr);
-            rho_out[i] = rho;
-            fp_out[i] = fpi;
+            // rho and F' arrays are optional: the force pass reads F' from rec
+            if (rho_out) rho_out[i] = rho;
+            if (fp_out) fp_out[i] = fpi;
             if (rec) rec[i] = make_int4(xi.x, xi.y, xi.z, __float_as_int(fpi));
             etot = (double)lerp2(F.x, F.y, fr) + 0.5 * (double)ep;
         }
@@ -433,9 +434,9 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
                                double* partial, void* stream) {
     (void)R;
     if (!reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
-        n_stride >= (1 << 28) || !rho ||
-        !fp || !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
-        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (const int*)rec == posq)
+        n_stride >= (1 << 28) || (!fp && !rec) ||
+        !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
+        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (rec && (const int*)rec == posq))
         return SIM_E_ARG;
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
@@ -501,3 +502,4 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
+

@@ -5,9 +5,9 @@ import torch
 from .. import _lib
 
 
-def backbone_outputs(mol, ff, virial=True):
+def backbone_outputs(mol, ff, virial=True, **kw):
     eng = mol.engine
-    ff.compute(eng, virial)
+    ff.compute(eng, virial, **kw)
     eng.reduce(0, 8, ff=ff)
     th = eng.thermo[0]
     return {

@@ -34,7 +34,7 @@ class EAMForceCUDAExt(BackboneInterface, nn.Module):
         self.ff = EAMForceField(eam_parser, n_r, n_rho, device=molecular.device, type_map=emap)
 
     def forward(self):
-        out = backbone_outputs(self.molecular, self.ff)
+        out = backbone_outputs(self.molecular, self.ff, keep_rho=True)
         # eam_force_cu.py:577-578: the embedding and pair parts of the energy
         emb = self.ff.embedding_energy(self.molecular.engine)[0]
         out["embedding_energy"] = emb

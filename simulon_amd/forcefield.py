@@ -108,16 +108,18 @@ class EAMForceField:
         Fi = (e[:, 0] + fr * (e[:, 1] - e[:, 0])).double()
         return torch.stack([Fi[eng.replica_slice(r)].sum() for r in range(eng.R)])
 
-    def compute(self, eng, virial=True, force=None, partial=None):
+    def compute(self, eng, virial=True, force=None, partial=None, keep_rho=False):
         """force/partial: optional output arrays in place of the engine's
-        (float [N][3] and double [ntiles][16]; the global-list kernels run)."""
-        rho = eng.scratch("eam_rho", torch.float32)
-        fp = eng.scratch("eam_fp", torch.float32)
+        (float [N][3] and double [ntiles][16]; the global-list kernels run).
+        keep_rho: also store the densities (embedding_energy reads them); the
+        global-list step writes only the records the force pass reads."""
         s = stream_ptr()
         own = force is None and partial is None
         force = eng.force if force is None else force
         partial = eng.partial if partial is None else partial
         if own and eng.use_tiled and self.params.ntypes == 1:
+            rho = eng.scratch("eam_rho", torch.float32)
+            fp = eng.scratch("eam_fp", torch.float32)
             fam = "rows" if eng.family == "rows" else "cell"
             common = (*(eng.row_args() if fam == "rows" else eng.cell_args()), self.params)
             call(f"sim_eam_density_{fam}", *common, ptr(rho), ptr(fp), ptr(eng.partial_cell),
@@ -127,8 +129,9 @@ class EAMForceField:
             return
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
+        rho = eng.scratch("eam_rho", torch.float32) if keep_rho else None
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), ptr(fp), ptr(rec),
+             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), None, ptr(rec),
              ptr(partial), s)
         call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(force), ptr(partial),
