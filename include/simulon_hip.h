@@ -4,8 +4,10 @@
  * Every entry point takes raw device pointers, element counts and a
  * hipStream_t (passed as void*), launches stream-ordered work only (no
  * device-wide synchronisation, no allocation), and returns 0 on success or a
- * negative SIM_E* code.  No entry point keeps global state: two calls on
- * different data may run concurrently on different streams.
+ * negative SIM_E* code.  No entry point keeps global state, except the one
+ * allocation-free entry point (sim_eam_energy_force_noalloc, with its fixed
+ * per-GPU work array): two calls of the others on different data may run
+ * concurrently on different streams.
  *
  * State is packed for R >= 1 independent systems ("replicas"): atoms of
  * replica r occupy [atom_base, atom_base + natoms) of every per-atom array,
@@ -62,6 +64,7 @@ extern "C" {
 #define SIM_E_ARG (-1)       /* bad argument (null pointer, size out of range) */
 #define SIM_E_LAUNCH (-2)    /* kernel launch failed */
 #define SIM_E_TYPES (-3)     /* too many atom types */
+#define SIM_E_SIZE (-4)      /* problem larger than an entry point's fixed work space */
 
 #define SIM_TILE 256         /* atoms per tile = threads per workgroup */
 #define SIM_MAX_TYPES 4
@@ -272,6 +275,28 @@ int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
                   const int* posq, const int* rec, const int* nbr, const int* nnbr,
                   int n_stride, sim_eam_params_t eam, float* force, double* partial,
                   int virial, void* stream);
+
+/* Allocation-free EAM evaluation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): sim_eam_density +
+ * sim_eam_force + sim_reduce of their columns [SIM_T_EPOT, SIM_T_KE) in one
+ * call, with the records, per-tile partials and reduction chunks in a
+ * fixed work array inside the library (one per GPU, held until the process
+ * ends) instead of caller buffers. The caller passes inputs and outputs:
+ * force[N][3] and thermo[R][16] (columns 0..7 written: energy, virial, W).
+ * rep_tiles: int[R+1] first tile of each replica (tiles grouped by replica).
+ * Limits: n_stride (atoms) <= SIM_NOALLOC_MAX_ATOMS, ntiles <=
+ * SIM_NOALLOC_MAX_TILES, R <= SIM_NOALLOC_MAX_REPLICAS, else SIM_E_SIZE; the
+ * work array is 36 MiB. Every call on one GPU uses that GPU's array: calls
+ * that overlap on one GPU (concurrent streams, or threads) race. Results
+ * equal those of the three entry points with caller buffers, bit for bit. */
+#define SIM_NOALLOC_MAX_ATOMS (1 << 21)
+#define SIM_NOALLOC_MAX_TILES 16384
+#define SIM_NOALLOC_MAX_REPLICAS 256
+int sim_eam_energy_force_noalloc(int R, const void* reps, const int* tiles, int ntiles,
+                                 const int* rep_tiles, const int* posq, const int* nbr,
+                                 const int* nnbr, int n_stride, sim_eam_params_t eam,
+                                 float* force, double* thermo, int virial, void* stream);
 
 /* Cell variants (one workgroup per cell slot, ncells = total cell
  * capacity): the 27-cell neighbourhood of the cell is staged in LDS column by

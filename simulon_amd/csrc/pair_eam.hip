@@ -18,6 +18,7 @@
 // linear interpolation needs for two functions: pass1[k] = (f(k), f(k+1),
 // phi(k), phi(k+1)), pass2[k] = (f'(k), f'(k+1), phi'(k), phi'(k+1)), with
 // the k+1 entry clamped at the last grid point as the reference clamps `next`.
+#include <mutex>
 #include <type_traits>
 
 #include "pair_common.h"
@@ -503,3 +504,65 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     return SIM_OK;
 }
 
+// ---- allocation-free EAM evaluation ---------------------------------------
+//
+// sim_eam_energy_force_noalloc keeps its work arrays (the pass-1 records with
+// F', the per-tile partial sums and the reduction chunks) in one file-scope
+// device array instead of buffers the caller allocates. Host code never uses
+// the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+namespace {
+
+constexpr size_t kNoallocRec = (size_t)SIM_NOALLOC_MAX_ATOMS * 4 * sizeof(int);
+constexpr size_t kNoallocPartial = (size_t)SIM_NOALLOC_MAX_TILES * SIM_NPART * sizeof(double);
+constexpr size_t kNoallocChunks =
+    (size_t)SIM_NOALLOC_MAX_REPLICAS * 64 * SIM_NPART * sizeof(double);   // sim_reduce's chunks
+
+static __device__ __align__(256) unsigned char scratch[kNoallocRec + kNoallocPartial +
+                                                       kNoallocChunks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call there), or nullptr.
+void* scratch_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
+}  // namespace
+
+extern "C" int sim_eam_energy_force_noalloc(int R, const void* reps, const int* tiles, int ntiles,
+                                            const int* rep_tiles, const int* posq, const int* nbr,
+                                            const int* nnbr, int n_stride, sim_eam_params_t eam,
+                                            float* force, double* thermo, int virial,
+                                            void* stream) {
+    if (R <= 0 || !rep_tiles || !thermo) return SIM_E_ARG;
+    if (n_stride > SIM_NOALLOC_MAX_ATOMS || ntiles > SIM_NOALLOC_MAX_TILES ||
+        R > SIM_NOALLOC_MAX_REPLICAS)
+        return SIM_E_SIZE;
+    unsigned char* base = static_cast<unsigned char*>(scratch_address());
+    if (!base) return SIM_E_LAUNCH;
+    int* rec = reinterpret_cast<int*>(base);
+    double* partial = reinterpret_cast<double*>(base + kNoallocRec);
+    double* chunks = reinterpret_cast<double*>(base + kNoallocRec + kNoallocPartial);
+    // pass 1 and pass 2 write every column [0, 8) of every tile's partial
+    // row, and the reduction writes every chunk it reads: nothing in the
+    // scratch array has to start at zero
+    int rc = sim_eam_density(R, reps, tiles, ntiles, posq, nbr, nnbr, n_stride, eam, nullptr,
+                             nullptr, rec, partial, stream);
+    if (rc != SIM_OK) return rc;
+    rc = sim_eam_force(R, reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
+                       virial, stream);
+    if (rc != SIM_OK) return rc;
+    return sim_reduce(R, rep_tiles, partial, SIM_T_EPOT, SIM_T_KE, chunks, thermo, stream);
+}
