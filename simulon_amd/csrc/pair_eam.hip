@@ -31,13 +31,16 @@ struct Lerp {
 };
 
 __device__ __forceinline__ Lerp lerp_r(float r, const sim_eam_params_t& E) {
-    const float rc = fminf(fmaxf(r, 0.0f), E.r_clamp);
+    const float rc = __builtin_amdgcn_fmed3f(r, 0.0f, E.r_clamp);   // clamp to [0, r_clamp]
     const float xf = rc * E.inv_dr;
     const int idx = (int)xf;
     return Lerp{idx, xf - (float)idx};
 }
 
-__device__ __forceinline__ float lerp2(float a, float b, float t) { return a + t * (b - a); }
+// a + t (b - a) with one rounding for the product and sum (fused)
+__device__ __forceinline__ float lerp2(float a, float b, float t) {
+    return __builtin_fmaf(t, b - a, a);
+}
 
 // Grid point k of a table pair (both values a linear interpolation needs, of
 // two functions): the interleaved float4 table in global memory, or (LT) the
@@ -146,9 +149,9 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
     // unconditional and in flight together; + 0.0f leaves sums unchanged
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
-        const float w = ((!MASKED || k + u < cnt) && r2[u] < thr) ? 1.0f : 0.0f;
-        rho += w * lerp2(v[u].x, v[u].y, L[u].frac);
-        ep += w * lerp2(v[u].z, v[u].w, L[u].frac);
+        const bool w = (!MASKED || k + u < cnt) && r2[u] < thr;
+        rho += w ? lerp2(v[u].x, v[u].y, L[u].frac) : 0.0f;
+        ep += w ? lerp2(v[u].z, v[u].w, L[u].frac) : 0.0f;
     }
 }
 
@@ -242,10 +245,13 @@ struct ForceAcc {
     float wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
 };
 
+// s / (r + 1e-8) of the reference is s / r in float32 for every r above
+// 0.17 A (1e-8 is below half an ulp of r there); inv_r = 1/r comes from the
+// one reciprocal square root that also gives r.
 template <bool VIRIAL>
 __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, float rz, float r,
-                                               float s) {
-    const float g = -s * __builtin_amdgcn_rcpf(r + 1e-8f);
+                                               float inv_r, float s) {
+    const float g = -s * inv_r;
     a.fx = __builtin_fmaf(g, rx, a.fx);
     a.fy = __builtin_fmaf(g, ry, a.fy);
     a.fz = __builtin_fmaf(g, rz, a.fz);
@@ -272,7 +278,7 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     const int nt = NT == 1 ? 1 : E.ntypes;
     int tj[UB];
     int4 xj[UB];
-    float rx[UB], ry[UB], rz[UB], r2[UB], r[UB];
+    float rx[UB], ry[UB], rz[UB], r2[UB], r[UB], ir[UB];
     Lerp L[UB];
     float4 va[UB], vb[UB];
 #pragma unroll
@@ -283,7 +289,9 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     for (int u = 0; u < UB; ++u) {
         qdisp<ORTHO>(xi, xj[u], B, rx[u], ry[u], rz[u]);
         r2[u] = sq3(rx[u], ry[u], rz[u]);
-        r[u] = pair_sqrt(r2[u]);
+        // masked self entries have r2 = 0: the floor keeps 1/r finite there
+        ir[u] = __builtin_amdgcn_rsqf(fmaxf(r2[u], 1e-20f));
+        r[u] = r2[u] * ir[u];
         L[u] = lerp_r(r[u], E);
     }
 #pragma unroll
@@ -297,12 +305,12 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     }
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
-        const float w = ((!MASKED || k + u < cnt) && r2[u] < thr) ? 1.0f : 0.0f;
+        const bool w = (!MASKED || k + u < cnt) && r2[u] < thr;
         const float dda = lerp2(va[u].x, va[u].y, L[u].frac);
         const float ddb = NT == 1 ? dda : lerp2(vb[u].x, vb[u].y, L[u].frac);
-        const float s = w * (fpi * dda + __int_as_float(xj[u].w) * ddb +
-                             lerp2(va[u].z, va[u].w, L[u].frac));
-        eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], s);
+        const float sv = __builtin_fmaf(fpi, dda, __builtin_fmaf(__int_as_float(xj[u].w), ddb,
+                                                                 lerp2(va[u].z, va[u].w, L[u].frac)));
+        eam_pair_force<VIRIAL>(a, rx[u], ry[u], rz[u], r[u], ir[u], w ? sv : 0.0f);
     }
 }
 

@@ -28,9 +28,12 @@ struct LJAcc {
 template <bool VIRIAL, int MODE>
 __device__ __forceinline__ void lj_pair(LJAcc& a, float rx, float ry, float rz, float r2, float thr,
                                         float eps, float sig, const sim_lj_params_t& lj) {
-    if (!(r2 < thr)) return;
+    // raw form: a pair outside the cutoff gets 1/r^2 = 0, which zeroes its
+    // energy, force and virial terms without a branch; the smoothed forms
+    // add terms that do not vanish with it, so they skip the pair
+    if (MODE != 0 && !(r2 < thr)) return;
     const float rc = lj.rc;
-    const float inv_r2 = __builtin_amdgcn_rcpf(r2);   // 1 ulp
+    const float inv_r2 = (MODE != 0 || r2 < thr) ? __builtin_amdgcn_rcpf(r2) : 0.0f;   // 1 ulp
     const float s2 = sig * sig * inv_r2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
