@@ -199,9 +199,10 @@ def replica_specs(nrep=64):
 
 
 def force_error_sample(eng, setfl, reps):
-    """float32 GPU forces vs the float64 oracle on the replicas `reps`."""
+    """float32 GPU forces vs the float64 oracle (oracle/cell_md.c) on the
+    replicas `reps`, at the positions the run ended with."""
+    from oracle import cell as ocell
     from oracle import eam as oeam
-    from oracle import neighbor as onb
     tab = oeam.EAMTables(oeam.EAMFile(setfl), dtype=np.float64)
     eng.build_list(force=True)
     eng.compute_forces(virial=True)
@@ -213,8 +214,7 @@ def force_error_sample(eng, setfl, reps):
         sl = eng.replica_slice(r)
         x = x_all[sl]
         H = eng.box(r)
-        ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(5.5, 1.0))
-        ref = oeam.eam_forward(x, ei, np.zeros(len(x), int), tab, H)["forces"]
+        ref = ocell.eam(x, H, None, tab, 1.0)["forces"]
         e = float(np.max(np.linalg.norm(f_all[sl] - ref, axis=1)) /
                   np.max(np.linalg.norm(ref, axis=1)))
         log(f"  force err replica {r} (N={len(x)}): {e:.3e}")
@@ -222,34 +222,66 @@ def force_error_sample(eng, setfl, reps):
     return worst
 
 
-def cpu_baseline_eam(setfl, natoms_side=10, seconds=15.0):
-    """The oracle's float32 EAM step (edge list built once, as between list
-    rebuilds) on one replica-sized cell, timed on the host."""
+def host_cpu():
+    """(CPU model name, threads this process may run on, os.cpu_count())."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, avail, os.cpu_count()
+
+
+def cpu_baseline_eam(setfl, seconds=15.0, nrep=64):
+    """The reference's CPU step in torch float32 (oracle/torch_ref.py: BAOB
+    Langevin, skin trigger, edge distances, edge-list EAM with scatter_add_)
+    on all the host threads this process may use, run replica by replica as
+    the reference's scan does, on a subsample of the scan (its smallest and
+    its median replica, about seconds/2 each), each on its half list built
+    once (as between list rebuilds)."""
     from oracle import eam as oeam
     from oracle import integrate as oint
     from oracle import neighbor as onb
-    pos, H = lat.w_bcc(natoms_side, seed=1, amp=0.02)
-    x = pos.astype(np.float32)
-    n = len(x)
+    from oracle import torch_ref as otr
+    model, avail, visible = host_cpu()
+    torch.set_num_threads(avail)
     tab = oeam.EAMTables(oeam.EAMFile(setfl), dtype=np.float32)
-    ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(5.5, 1.0))
-    m = oint.internal_mass(np.full(n, lat.W_MASS))
-    rng = np.random.default_rng(0)
-    v = oint.maxwell_boltzmann(np.full(n, lat.W_MASS), 300.0, rng)
-    f = oeam.eam_forward(x, ei, np.zeros(n, int), tab, H)["forces"]
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        vh, x = oint.first_half(x, v, f, m, 0.001)
-        f = oeam.eam_forward(x, ei, np.zeros(n, int), tab, H)["forces"]
-        v, _, _ = oint.second_half(vh, f, m, 0.001, gamma=10.0, T=300.0,
-                                   noise=rng.standard_normal((n, 3)).astype(np.float32))
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": round(steps * n / dt, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle float32 EAM + BAOB step, {n}-atom W bcc cell, {steps} steps in "
-                      f"{dt:.1f} s, list built once (host {os.cpu_count()} cores visible, numpy "
-                      f"single-threaded)"}
+    rs = lat.replica_set(nrep)
+    sizes = np.array([len(p) for p, _, _ in rs])
+    pick = [int(np.argmin(sizes)), int(np.argsort(sizes)[len(sizes) // 2])]
+    work, el, parts = 0, 0.0, []
+    for r in pick:
+        pos, H, T = rs[r]
+        x = pos.astype(np.float32)
+        n = len(x)
+        ei, _ = onb.cell_half_list(x, H, onb.list_radius_sq(5.5, 1.0))
+        v = oint.maxwell_boltzmann(np.full(n, lat.W_MASS), T, np.random.default_rng(r))
+        st = otr.TorchEAMStep(x, v, lat.W_MASS, H, ei, tab, dt=0.001, skin=1.0, gamma=10.0, T=T,
+                              seed=r)
+        st.step()                                   # warm-up
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds / len(pick):
+            st.step()
+            steps += 1
+        dt = time.perf_counter() - t0
+        work += n * steps
+        el += dt
+        parts.append(f"replica {r} ({n} atoms, {T:.0f} K): {steps} steps in {dt:.1f} s")
+    return {"value": round(work / el, 1), "unit": "atom-steps/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": "the reference's CPU step in torch float32 (BAOB Langevin + skin trigger + "
+                      "edge distances + edge-list EAM with scatter_add_), replica by replica: "
+                      + "; ".join(parts) + "; each on its half list built once",
+            "cpu_model": model, "os_cpu_count": visible, "torch_threads": torch.get_num_threads()}
 
 
 def cpu_baseline_lj(pos, H, dt, seconds=15.0, mode="raw"):
@@ -300,15 +332,19 @@ def bench_replicas(args):
                         family=args.family)
     eng.force_field = EAMForceField(setfl, device=eng.device)
     eng.setup()
-    eng.run(args.warmup, graph=True, virial=False)
-    t = timed(lambda k: eng.run(k, graph=True, virial=False, check=False), args.steps)
+    # every step computes the virial tensor, as the reference's does
+    # (core/md_model.py:146-162, core/force/eam_force_cu.py:566-570)
+    eng.run(args.warmup, graph=True, virial=True)
+    b0 = eng.nbuild.cpu().numpy().copy()
+    t = timed(lambda k: eng.run(k, graph=True, virial=True, check=False), args.steps)
     eng.check()
     log(f"pair kernels: {eng.family} (LDS cap {eng.cap}); autotune {eng.profile.get('autotune_ms')}")
     batched = natoms.sum() * args.steps / t
-    builds = eng.nbuild.cpu().numpy()
+    builds = eng.nbuild.cpu().numpy() - b0
     log(f"batched: {t * 1e3 / args.steps:.3f} ms/step, {batched:.4e} atom-steps/s, "
-        f"rebuilds per replica min {builds.min()} max {builds.max()}")
-    roof = eam_kernel_roofline(eng, virial=False)      # the step above runs without it
+        f"rebuilds per replica in the timed steps: min {builds.min()} max {builds.max()} "
+        f"total {builds.sum()}")
+    roof = eam_kernel_roofline(eng, virial=True)       # the instance the step launches
     log(f"roofline {roof}")
     # plain per-replica loop on the same GPU
     plain_t = 0.0
@@ -317,16 +353,16 @@ def bench_replicas(args):
         e1 = ReplicaEngine([s], [lat.W_MASS], cutoff=5.5, skin=args.skin, seed=2024 + k)
         e1.force_field = eng.force_field
         e1.setup()
-        e1.run(2, graph=True, virial=False)
-        plain_t += timed(lambda k_: e1.run(k_, graph=True, virial=False, check=False), plain_steps)
+        e1.run(2, graph=True, virial=True)
+        plain_t += timed(lambda k_: e1.run(k_, graph=True, virial=True, check=False), plain_steps)
         e1.check()
         del e1
     plain = natoms.sum() * plain_steps / plain_t
     log(f"plain loop: {plain_t * 1e3 / plain_steps:.3f} ms per all-replica step, {plain:.4e} "
         f"atom-steps/s, speedup {batched / plain:.2f}x")
-    small = int(np.argmin(natoms))
-    mid = int(np.argsort(natoms)[len(natoms) // 2])
-    ferr = force_error_sample(eng, setfl, [small, mid]) if not args.no_check else None
+    order = np.argsort(natoms)
+    sample = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    ferr = force_error_sample(eng, setfl, sample) if not args.no_check else None
     out = {
         "metric": METRIC, "value": round(float(batched), 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -341,13 +377,18 @@ def bench_replicas(args):
         "speedup_vs_plain": round(float(batched / plain), 3),
         "ns_per_day": round(args.steps * 0.001e-3 * 86400 / t, 3),
         "force_err_vs_fp64": ferr,
-        "rebuilds_per_replica": [int(builds.min()), int(builds.max())],
+        "force_err_replicas": [int(natoms[r]) for r in sample],
+        "rebuilds_in_timed_steps": {"per_replica_min": int(builds.min()),
+                                    "per_replica_max": int(builds.max()),
+                                    "total": int(builds.sum())},
+        "virial": "every step (tensor)",
         "roofline": roof,
         "step_roofline": step_roofline(int(natoms.sum()), t * 1e3 / args.steps,
                                        132 + 8 * roof["nbar_full"]),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_eam(setfl, seconds=args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline_eam(setfl, seconds=args.cpu_seconds,
+                                               nrep=args.replicas)
     return out
 
 
