@@ -109,14 +109,43 @@ __device__ __forceinline__ void dense(f4v (&out)[MT], const unsigned short* W, i
     }
 }
 
-// Radial basis of one lane: Gaussian centres of its 8 features.
+// Radial basis of one lane: Gaussian centres of its 8 features, two runs
+// of 4 consecutive centres (mu[0..3] and mu[4..7]).
 struct Basis {
     float mu[8];
     float alpha;    // log2(e) / (2 w^2)
     float inv_w2;
     float inv_rc;
     float thr;      // r2 < thr  <=>  sqrtf(r2) < rc
+    float dmu;      // centre spacing
+    float two_ad;   // 2 alpha dmu
+    float ad2;      // alpha dmu^2
+    float c2;       // 2^(-2 alpha dmu^2)
 };
+
+// The lane's 8 Gaussians exp(-(r - mu_k)^2 / (2 w^2)) and x_k = r - mu_k.
+// Within a run of equally spaced centres the ratio of neighbours is
+//   G_{k+1} / G_k = 2^(alpha (2 dmu x_k - dmu^2)) = R_k,   R_{k+1} = R_k 2^(-2 alpha dmu^2),
+// so a run of 4 costs 2 exponentials and 5 products instead of 4
+// exponentials (relative error a few float32 ulps; Gaussians below the
+// float32 range come out as 0, as they do when evaluated one by one).
+__device__ __forceinline__ void gauss8(float r, const Basis& b, float (&G)[8], float (&x)[8]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float x0 = r - b.mu[4 * h];
+        float g = __builtin_amdgcn_exp2f(-b.alpha * x0 * x0);
+        float R = __builtin_amdgcn_exp2f(__builtin_fmaf(b.two_ad, x0, -b.ad2));
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            G[4 * h + m] = g;
+            x[4 * h + m] = x0 - (float)m * b.dmu;
+            if (m < 3) {
+                g = g * R;
+                R = R * b.c2;
+            }
+        }
+    }
+}
 
 __device__ __forceinline__ Basis make_basis(const sim_mlp_net_t& net, int q, float thr) {
     Basis b;
@@ -129,6 +158,10 @@ __device__ __forceinline__ Basis make_basis(const sim_mlp_net_t& net, int q, flo
     b.inv_w2 = net.inv_w2;
     b.inv_rc = 1.0f / net.rc;
     b.thr = thr;
+    b.dmu = net.dmu;
+    b.two_ad = 2.0f * b.alpha * net.dmu;
+    b.ad2 = b.alpha * net.dmu * net.dmu;
+    b.c2 = __builtin_amdgcn_exp2f(-2.0f * b.ad2);
     return b;
 }
 
@@ -195,11 +228,10 @@ __device__ __forceinline__ void descriptors(float (&g)[8], int i, int cmax, cons
             const float r = sqrtf(r2);
             float fc, dfc;
             cutoff_fn(r, b, fc, dfc);
+            float G[8], x[8];
+            gauss8(r, b, G, x);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const float x = r - b.mu[v];
-                g[v] = __builtin_fmaf(__builtin_amdgcn_exp2f(-b.alpha * x * x), fc, g[v]);
-            }
+            for (int v = 0; v < 8; ++v) g[v] = __builtin_fmaf(G[v], fc, g[v]);
         }
     });
 }
@@ -325,13 +357,13 @@ __device__ __forceinline__ void force_atom(float (&acc)[10], int i, int cmax, co
             float fc, dfc;
             cutoff_fn(r, b, fc, dfc);
             const float fcw = fc * b.inv_w2;
+            float G[8], x[8];
+            gauss8(r, b, G, x);
             float s = 0.f;   // this lane's share of dE/dr
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                const float x = r - b.mu[v];
-                const float G = __builtin_amdgcn_exp2f(-b.alpha * x * x);
                 const float cc = v < 4 ? ci0[v] + cj0[v] : ci1[v - 4] + cj1[v - 4];
-                s = __builtin_fmaf(cc * G, dfc - x * fcw, s);
+                s = __builtin_fmaf(cc * G[v], dfc - x[v] * fcw, s);
             }
             const float fr = -s * __builtin_amdgcn_rcpf(r);
             acc[0] = __builtin_fmaf(fr, rx, acc[0]);
@@ -424,12 +456,10 @@ __device__ __forceinline__ void adjoint_atom(float (&u)[8], int i, int cmax, con
             float fc, dfc;
             cutoff_fn(r, b, fc, dfc);
             const float fcw = fc * b.inv_w2;
+            float G[8], x[8];
+            gauss8(r, b, G, x);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const float x = r - b.mu[v];
-                const float G = __builtin_amdgcn_exp2f(-b.alpha * x * x);
-                u[v] = __builtin_fmaf(-G * (dfc - x * fcw), proj, u[v]);
-            }
+            for (int v = 0; v < 8; ++v) u[v] = __builtin_fmaf(-G[v] * (dfc - x[v] * fcw), proj, u[v]);
         }
     });
 }
