@@ -54,6 +54,11 @@ def test_minimizer_relaxes_to_lattice(method):
                                              force_threshold=1e-3)
         hist = np.array(model.minimize_history)
         assert np.all(np.diff(hist[:, 0]) < 0)          # every accepted step lowers E
+        # one host read per evaluation; an accepted first trial is the next
+        # step's base point, so reads stay close to one per accepted step
+        steps = len(hist)
+        print(f"steepest: {steps} steps, {model.minimize_reads} host reads", flush=True)
+        assert model.minimize_reads <= 1.25 * steps + 2
         tol = 0.05
     else:
         e = minimize_energy_bfgs_scipy(model, max_iter=100)
