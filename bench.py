@@ -193,9 +193,21 @@ def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
 
 
 # ---------------------------------------------------------------------------
-def replica_specs(nrep=64):
-    return [ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT", temperature=T, gamma=10.0)
-            for pos, H, T in lat.replica_set(nrep)]
+def replica_specs(nrep=64, state=None):
+    """The 64-replica scan; state: an npz written by --state-out (positions and
+    velocities of a finished run, caller order) to start from instead of the
+    lattices."""
+    specs = [ReplicaSpec(positions=pos, box=H, dt=0.001, ensemble="NVT", temperature=T,
+                         gamma=10.0) for pos, H, T in lat.replica_set(nrep)]
+    if state is not None:
+        st = np.load(state)
+        base = np.concatenate([[0], np.cumsum([len(s.positions) for s in specs])])
+        if int(base[-1]) != len(st["pos"]):
+            raise SystemExit(f"{state}: {len(st['pos'])} atoms, the scan has {base[-1]}")
+        for r, s in enumerate(specs):
+            s.positions = st["pos"][base[r]:base[r + 1]]
+            s.velocities = st["vel"][base[r]:base[r + 1]]
+    return specs
 
 
 def force_error_sample(eng, setfl, reps):
@@ -325,7 +337,7 @@ def cpu_baseline_lj(pos, H, dt, seconds=15.0, mode="raw"):
 def bench_replicas(args):
     tmp = tempfile.mkdtemp()
     setfl = lat.write_w_synthetic_setfl(os.path.join(tmp, "W_synthetic.eam.alloy"))
-    specs = replica_specs(args.replicas)
+    specs = replica_specs(args.replicas, args.state_in)
     natoms = np.array([len(s.positions) for s in specs])
     log(f"replicas {len(specs)} atoms total {natoms.sum()} min {natoms.min()} max {natoms.max()}")
     eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=args.skin, seed=2024,
@@ -344,6 +356,10 @@ def bench_replicas(args):
     log(f"batched: {t * 1e3 / args.steps:.3f} ms/step, {batched:.4e} atom-steps/s, "
         f"rebuilds per replica in the timed steps: min {builds.min()} max {builds.max()} "
         f"total {builds.sum()}")
+    if args.state_out:
+        np.savez(args.state_out, pos=eng.to_caller(eng.positions(unwrap=True)).cpu().numpy(),
+                 vel=eng.to_caller(eng.vel).cpu().numpy())
+        log(f"state after the timed steps written to {args.state_out}")
     roof = eam_kernel_roofline(eng, virial=True)       # the instance the step launches
     log(f"roofline {roof}")
     # plain per-replica loop on the same GPU
@@ -608,6 +624,10 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy a device buffer of this many MiB five times (a kernel "
                          "of known traffic, to scale hardware byte counters)")
+    ap.add_argument("--state-out", default=None,
+                    help="w_eam_replicas: write positions and velocities after the timed steps")
+    ap.add_argument("--state-in", default=None,
+                    help="w_eam_replicas: start from a --state-out file (profiling the end state)")
     ap.add_argument("--mlp-frames", type=int, default=2000)
     ap.add_argument("--mlp-epochs", type=int, default=200)
     ap.add_argument("--mlp-threshold", type=float, default=0.1)

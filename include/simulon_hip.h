@@ -276,6 +276,20 @@ int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
                   int n_stride, sim_eam_params_t eam, float* force, double* partial,
                   int virial, void* stream);
 
+/* sim_eam_force fused with sim_integrate_second (same arguments as the two,
+ * partial rows = tiles): with atom i's force in registers the pass also does
+ * the second half of the step for it (O + B, the arithmetic of
+ * sim_integrate_second, bit for bit) and writes the kinetic columns
+ * SIM_T_KE..SIM_T_KE+6 of partial, so the step needs no separate
+ * second-half kernel. Replaces core/integrator/integrator.py:102-124 after
+ * the EAM force pass of core/force/eam_force_cu.py:555-570. */
+int sim_eam_force_second(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
+                         const int* rec, const int* nbr, const int* nnbr, int n_stride,
+                         sim_eam_params_t eam, float* force, double* partial, int virial,
+                         sim_types_t types, const float* vel_half, float* vel, const float* noise,
+                         const long long* key, unsigned long long seed,
+                         const unsigned long long* step_dev, void* stream);
+
 /* Allocation-free EAM evaluation: no counterpart in the reference. For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc): sim_eam_density +

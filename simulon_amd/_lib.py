@@ -113,6 +113,8 @@ SIGNATURES = {
                            _vp]),
     "sim_eam_energy_force_noalloc": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams,
                                           _vp, _vp, _I, _vp]),
+    "sim_eam_force_second": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
+                                  _I, Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_first_q": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,

@@ -73,45 +73,14 @@ __global__ void __launch_bounds__(SIM_TILE)
     const SimTile t = tile_of(tiles);
     const sim_replica_t& P = reps[t.rep];
     const int i = t.start + threadIdx.x;
-    double ke = 0.0, kxx = 0.0, kyy = 0.0, kzz = 0.0, kxy = 0.0, kxz = 0.0, kyz = 0.0;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < P.atom_base + P.natoms) {
         // one type: no need to read the position record for it
         const int ty_i = ty.ntypes == 1 ? 0 : (int)pos[i].w;
-        const float m = ty.mass[ty_i];
-        const float dt = P.dtf;
         const size_t o = (size_t)i * 3;
-        float v0 = vh[o], v1 = vh[o + 1], v2 = vh[o + 2];
-        if (P.langevin) {
-            float g0, g1, g2;
-            if (noise) {
-                g0 = noise[o];
-                g1 = noise[o + 1];
-                g2 = noise[o + 2];
-            } else {
-                const uint32_t id = key ? (uint32_t)(key[i] - P.atom_base) : (uint32_t)i;
-                gauss3(seed ^ P.noise_key, step, id, g0, g1, g2);
-            }
-            const float s = P.rff[ty_i];
-            v0 = v0 * P.damp + g0 * s;
-            v1 = v1 * P.damp + g1 * s;
-            v2 = v2 * P.damp + g2 * s;
-        }
-        v0 = v0 + (0.5f * (force[o] / m)) * dt;
-        v1 = v1 + (0.5f * (force[o + 1] / m)) * dt;
-        v2 = v2 + (0.5f * (force[o + 2] / m)) * dt;
-        vel[o] = v0;
-        vel[o + 1] = v1;
-        vel[o + 2] = v2;
-        const double md = (double)m, a = v0, b = v1, c = v2;
-        ke = 0.5 * md * ((a * a + b * b) + c * c);
-        kxx = md * a * a;
-        kyy = md * b * b;
-        kzz = md * c * c;
-        kxy = md * a * b;
-        kxz = md * a * c;
-        kyz = md * b * c;
+        second_half_atom(P, i, ty_i, ty, vh, force[o], force[o + 1], force[o + 2], vel, noise,
+                         key, seed, step, v);
     }
-    double v[7] = {ke, kxx, kyy, kzz, kxy, kxz, kyz};
     block_sum<7>(v, lds);
     if (threadIdx.x == 0) {
         double* p = partial + (size_t)blockIdx.x * SIM_NPART;

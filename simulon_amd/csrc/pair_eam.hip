@@ -6,7 +6,7 @@
 //           E_i   = F_ti(rho_i) + 1/2 sum_j phi_{ti,tj}(r_ij);  fp_i = F'_ti(rho_i)
 //   pass 2  s_ij  = fp_i f'_{ti,tj}(r) + fp_j f'_{tj,ti}(r) + phi'_{ti,tj}(r)
 //           F_i   = sum_j -s_ij rij / (r + 1e-8)
-//           virial = 1/2 sum s r ; W = 1/2 sum (-s/(r+1e-8)) rij (x) rij
+//           virial = 1/2 sum s r = -tr W ; W = 1/2 sum (-s/(r+1e-8)) rij (x) rij
 // The reference computes rho with 2 float atomics per edge and forces with 6
 // ("cuda source/eam_cuda_ext_kernel.cu:47-52, 108-113"); here each thread owns
 // one atom of the full list, so both passes are register sums. Atoms are
@@ -256,7 +256,9 @@ __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, 
     a.fy = __builtin_fmaf(g, ry, a.fy);
     a.fz = __builtin_fmaf(g, rz, a.fz);
     if (VIRIAL) {
-        a.vir = __builtin_fmaf(s, r, a.vir);
+        // the scalar virial sum s r = -sum g r^2 is minus the trace of the
+        // tensor, taken once per atom in the epilogue
+        (void)r;
         const float gx = g * rx, gy = g * ry;
         a.wxx = __builtin_fmaf(gx, rx, a.wxx);
         a.wyy = __builtin_fmaf(gy, ry, a.wyy);
@@ -348,15 +350,31 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
     }
 }
 
-template <int NT, bool LT, bool VIRIAL, int TPW>
+// The integrator's second half fused into the force pass (INTEG): with the
+// force of atom i in registers the pass finishes the step for it (O + B, the
+// arithmetic of k_second) and writes the kinetic columns of its tile's
+// partial row, instead of a separate kernel re-reading forces and v_half.
+struct SecondHalf {
+    const float* vh;
+    float* vel;
+    const float* noise;
+    const long long* key;
+    unsigned long long seed;
+    const unsigned long long* step_dev;
+    sim_types_t ty;
+};
+
+template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG>
 __global__ void __launch_bounds__(SIM_TILE * TPW)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                 int ntiles, const int4* __restrict__ typ, const int4* __restrict__ rec,
                 const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
                 sim_eam_params_t E, float* __restrict__ force, double* __restrict__ partial,
-                float thr) {
+                float thr, SecondHalf S) {
     extern __shared__ float2 tl[];
-    __shared__ double lds[TPW * 4 * 7];
+    constexpr int NV = INTEG ? 14 : 7;
+    __shared__ double lds[TPW * 4 * NV];
+    const unsigned long long step = (INTEG && S.step_dev) ? S.step_dev[0] : 0ull;
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass2), E.n_r);
     for (TileWalk w(LT, ntiles, TPW); w.m < w.end; w.m += w.step) {
@@ -383,13 +401,33 @@ __global__ void __launch_bounds__(SIM_TILE * TPW)
             force[(size_t)i * 3 + 1] = a.fy;
             force[(size_t)i * 3 + 2] = a.fz;
         }
-        double v[7] = {0.5 * a.vir, 0.5 * a.wxx, 0.5 * a.wyy, 0.5 * a.wzz,
-                       0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
-        group_sum<7>(v, lds, g, lt);
+        double v[NV];
+        v[0] = -0.5 * (((double)a.wxx + a.wyy) + a.wzz);
+        v[1] = 0.5 * a.wxx;
+        v[2] = 0.5 * a.wyy;
+        v[3] = 0.5 * a.wzz;
+        v[4] = 0.5 * a.wxy;
+        v[5] = 0.5 * a.wxz;
+        v[6] = 0.5 * a.wyz;
+        if (INTEG) {
+            double kin[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (active) {
+                const int ty_i = S.ty.ntypes == 1 ? 0 : typ[i].w;
+                second_half_atom(P, i, ty_i, S.ty, S.vh, a.fx, a.fy, a.fz, S.vel, S.noise, S.key,
+                                 S.seed, step, kin);
+            }
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v[7 + k] = kin[k];
+        }
+        group_sum<NV>(v, lds, g, lt);
         if (have && lt == 0) {
             double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
             for (int k = 0; k < 7; ++k) p[SIM_T_VIRIAL + k] = v[k];
+            if (INTEG) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k) p[SIM_T_KE + k] = v[7 + k];
+            }
         }
         __syncthreads();   // lds is reused by the next tiles
     }
@@ -471,16 +509,16 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     return SIM_OK;
 }
 
-extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
-                             const int* posq, const int* rec, const int* nbr, const int* nnbr,
-                             int n_stride, sim_eam_params_t eam, float* force, double* partial,
-                             int virial, void* stream) {
-    (void)R;
+static int force_impl(const void* reps, const int* tiles, int ntiles, const int* posq,
+                      const int* rec, const int* nbr, const int* nnbr, int n_stride,
+                      sim_eam_params_t eam, float* force, double* partial, int virial,
+                      const SecondHalf* S2, void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !posq || !rec || !nbr || !nnbr || n_stride <= 0 ||
         n_stride >= (1 << 28) ||
         !force || !partial || !eam.pass2 || eam.n_r < 2 || eam.ntypes < 1 ||
         eam.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;
+    const SecondHalf S = S2 ? *S2 : SecondHalf{};
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
@@ -489,27 +527,56 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
     hipStream_t st = as_stream(stream);
     const dim3 g(ntiles), b(SIM_TILE);
     const size_t tb = lds_tab_bytes(eam);
-    auto go = [&](auto v) {
+    auto go = [&](auto v, auto in) {
         constexpr bool V = decltype(v)::value;
+        constexpr bool I = decltype(in)::value;
         constexpr int TPW = SIM_EAM_FORCE_TPW;
-        static const bool lds_ok = allow_lds(k_eam_force<1, true, V, TPW>);
+        static const bool lds_ok = allow_lds(k_eam_force<1, true, V, TPW, I>);
         if (use_lds_tab(eam, ntiles) && lds_ok)
-            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW>),
+            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW, I>),
                                dim3(persistent_grid((ntiles + TPW - 1) / TPW,
                                                     2 / TPW > 0 ? 2 / TPW : 1)),
                                dim3(SIM_TILE * TPW), tb, st, P, T, ntiles, X, Q, nbr, nnbr,
-                               n_stride, eam, force, partial, thr);
+                               n_stride, eam, force, partial, thr, S);
         else if (eam.ntypes == 1)
-            hipLaunchKernelGGL((k_eam_force<1, false, V, 1>), g, b, 0, st, P, T, ntiles, X, Q,
-                               nbr, nnbr, n_stride, eam, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<1, false, V, 1, I>), g, b, 0, st, P, T, ntiles, X, Q,
+                               nbr, nnbr, n_stride, eam, force, partial, thr, S);
         else
-            hipLaunchKernelGGL((k_eam_force<0, false, V, 1>), g, b, 0, st, P, T, ntiles, X, Q,
-                               nbr, nnbr, n_stride, eam, force, partial, thr);
+            hipLaunchKernelGGL((k_eam_force<0, false, V, 1, I>), g, b, 0, st, P, T, ntiles, X, Q,
+                               nbr, nnbr, n_stride, eam, force, partial, thr, S);
     };
-    if (virial) go(std::true_type());
-    else go(std::false_type());
+    if (S2) {
+        if (virial) go(std::true_type(), std::true_type());
+        else go(std::false_type(), std::true_type());
+    } else {
+        if (virial) go(std::true_type(), std::false_type());
+        else go(std::false_type(), std::false_type());
+    }
     SIM_CHECK_LAUNCH();
     return SIM_OK;
+}
+
+extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
+                             const int* posq, const int* rec, const int* nbr, const int* nnbr,
+                             int n_stride, sim_eam_params_t eam, float* force, double* partial,
+                             int virial, void* stream) {
+    (void)R;
+    return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
+                      virial, nullptr, stream);
+}
+
+extern "C" int sim_eam_force_second(int R, const void* reps, const int* tiles, int ntiles,
+                                    const int* posq, const int* rec, const int* nbr,
+                                    const int* nnbr, int n_stride, sim_eam_params_t eam,
+                                    float* force, double* partial, int virial, sim_types_t types,
+                                    const float* vel_half, float* vel, const float* noise,
+                                    const long long* key, unsigned long long seed,
+                                    const unsigned long long* step_dev, void* stream) {
+    (void)R;
+    if (!vel_half || !vel || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES) return SIM_E_ARG;
+    const SecondHalf S{vel_half, vel, noise, key, seed, step_dev, types};
+    return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
+                      virial, &S, stream);
 }
 
 // ---- allocation-free EAM evaluation ---------------------------------------

@@ -168,3 +168,51 @@ __device__ __forceinline__ void gauss3(unsigned long long seed, unsigned long lo
     g2 = rb * cb;
     (void)sb;
 }
+
+// Second half of the velocity-Verlet step for atom i (core/integrator/
+// integrator.py:102-124): v = vh damp + xi rff (Langevin replicas), then
+// v += (0.5 F/m) dt, all float32; writes vel and returns the kinetic terms
+// (1/2 m v^2, m v_a v_b for xx yy zz xy xz yz) in float64. Shared by
+// k_second and the EAM force pass that fuses it, so both give the same bits.
+// noise: optional float[N][3] of N(0,1) draws (else Philox keyed by the
+// replica's key, the atom's index within its replica and the step).
+__device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, int ty_i,
+                                                 const sim_types_t& ty, const float* vh,
+                                                 float f0, float f1, float f2, float* vel,
+                                                 const float* noise, const long long* key,
+                                                 unsigned long long seed,
+                                                 unsigned long long step, double (&k)[7]) {
+    const float m = ty.mass[ty_i];
+    const float dt = P.dtf;
+    const size_t o = (size_t)i * 3;
+    float v0 = vh[o], v1 = vh[o + 1], v2 = vh[o + 2];
+    if (P.langevin) {
+        float g0, g1, g2;
+        if (noise) {
+            g0 = noise[o];
+            g1 = noise[o + 1];
+            g2 = noise[o + 2];
+        } else {
+            const uint32_t id = key ? (uint32_t)(key[i] - P.atom_base) : (uint32_t)i;
+            gauss3(seed ^ P.noise_key, step, id, g0, g1, g2);
+        }
+        const float s = P.rff[ty_i];
+        v0 = v0 * P.damp + g0 * s;
+        v1 = v1 * P.damp + g1 * s;
+        v2 = v2 * P.damp + g2 * s;
+    }
+    v0 = v0 + (0.5f * (f0 / m)) * dt;
+    v1 = v1 + (0.5f * (f1 / m)) * dt;
+    v2 = v2 + (0.5f * (f2 / m)) * dt;
+    vel[o] = v0;
+    vel[o + 1] = v1;
+    vel[o + 2] = v2;
+    const double md = (double)m, a = v0, b = v1, c = v2;
+    k[0] = 0.5 * md * ((a * a + b * b) + c * c);
+    k[1] = md * a * a;
+    k[2] = md * b * b;
+    k[3] = md * c * c;
+    k[4] = md * a * b;
+    k[5] = md * a * c;
+    k[6] = md * b * c;
+}

@@ -531,16 +531,30 @@ class ReplicaEngine:
         call("sim_pos_quantize", ptr(self.reps), ptr(self.tiles), self.ntiles, ptr(self.pos4),
              ptr(self.posq), stream_ptr())
 
-    def build_and_forces(self, virial):
+    def build_and_forces(self, virial, noise=None, second=False):
         """Gated rebuild and forces right after first_half(records=True): the
         records of the new positions are current (written by the first half,
-        rewritten by a reorder), so the force pass does not quantize again."""
+        rewritten by a reorder), so the force pass does not quantize again.
+        second=True: the step's second half too (noise as in second_half), in
+        the force pass where the force field fuses it, else after it."""
         self.build_list(force=False)
         self._posq_fresh = True
         try:
-            self.compute_forces(virial=virial)
+            fused = False
+            if second and getattr(self.force_field, "fuses_second", False):
+                fused = self.force_field.compute(self, virial, second=self.second_args(noise))
+            else:
+                self.compute_forces(virial=virial)
         finally:
             self._posq_fresh = False
+        if second and not fused:
+            self.second_half(noise, advance=False)
+
+    def second_args(self, noise=None):
+        """Arguments of the second half after the force arrays: types, v_half,
+        v, noise, atom keys, seed, device step counter."""
+        return (self.types_struct, ptr(self.vel_half), ptr(self.vel), ptr(noise), ptr(self.order),
+                0, ptr(self.step_ctr))
 
     def compute_forces(self, virial=True):
         if self.force_field is None:
@@ -656,8 +670,7 @@ class ReplicaEngine:
         if self.loading:
             self.deform()
         self.first_half(records=True)
-        self.build_and_forces(virial or self.has_barostat)
-        self.second_half(noise, advance=False)
+        self.build_and_forces(virial or self.has_barostat, noise, second=True)
         self.reduce(0, 15, advance=True)
         if self.has_barostat:
             self.barostat()

@@ -108,11 +108,16 @@ class EAMForceField:
         Fi = (e[:, 0] + fr * (e[:, 1] - e[:, 0])).double()
         return torch.stack([Fi[eng.replica_slice(r)].sum() for r in range(eng.R)])
 
-    def compute(self, eng, virial=True, force=None, partial=None, keep_rho=False):
+    fuses_second = True
+
+    def compute(self, eng, virial=True, force=None, partial=None, keep_rho=False, second=None):
         """force/partial: optional output arrays in place of the engine's
         (float [N][3] and double [ntiles][16]; the global-list kernels run).
         keep_rho: also store the densities (embedding_energy reads them); the
-        global-list step writes only the records the force pass reads."""
+        global-list step writes only the records the force pass reads.
+        second: the engine's second_args(); the global-list force pass then
+        also does the step's second half (sim_eam_force_second). Returns
+        whether it did."""
         s = stream_ptr()
         own = force is None and partial is None
         force = eng.force if force is None else force
@@ -126,13 +131,18 @@ class EAMForceField:
                  ptr(eng.overflow), s)
             call(f"sim_eam_force_{fam}", *common, ptr(fp), ptr(eng.force), ptr(eng.partial_cell),
                  int(bool(virial)), ptr(eng.overflow), s)
-            return
+            return False
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
         rho = eng.scratch("eam_rho", torch.float32) if keep_rho else None
         call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
              ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), None, ptr(rec),
              ptr(partial), s)
-        call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-             ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(force), ptr(partial),
-             int(bool(virial)), s)
+        args = (eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq), ptr(rec),
+                ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(force), ptr(partial),
+                int(bool(virial)))
+        if second is not None and own:
+            call("sim_eam_force_second", *args, *second, s)
+            return True
+        call("sim_eam_force", *args, s)
+        return False
