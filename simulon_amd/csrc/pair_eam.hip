@@ -114,6 +114,22 @@ struct TileWalk {
 #ifndef SIM_EAM_FORCE_TPW
 #define SIM_EAM_FORCE_TPW 4
 #endif
+// Workgroups per CU of the persistent density and force kernels (0: 2 / TPW,
+// at least 1) and the waves per SIMD their registers are held to (launch
+// bound; 1: no bound).
+#ifndef SIM_EAM_DENSITY_WG_PER_CU
+#define SIM_EAM_DENSITY_WG_PER_CU 0
+#endif
+#ifndef SIM_EAM_FORCE_WG_PER_CU
+#define SIM_EAM_FORCE_WG_PER_CU 0
+#endif
+#ifndef SIM_EAM_DENSITY_WAVES
+#define SIM_EAM_DENSITY_WAVES 1
+#endif
+#ifndef SIM_EAM_FORCE_WAVES
+#define SIM_EAM_FORCE_WAVES 1
+#endif
+constexpr int eam_wg_per_cu(int set, int tpw) { return set > 0 ? set : (2 / tpw > 0 ? 2 / tpw : 1); }
 
 // ---- pass 1 ---------------------------------------------------------------
 // One batch of UB list entries starting at k (indices in j[]). MASKED: the
@@ -186,7 +202,7 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
 }
 
 template <int NT, bool LT, int TPW>
-__global__ void __launch_bounds__(SIM_TILE * TPW)
+__global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                   int ntiles, const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
@@ -365,7 +381,7 @@ struct SecondHalf {
 };
 
 template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG>
-__global__ void __launch_bounds__(SIM_TILE * TPW)
+__global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                 int ntiles, const int4* __restrict__ typ, const int4* __restrict__ rec,
                 const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
@@ -496,7 +512,8 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     if (use_lds_tab(eam, ntiles) && lds_ok)
         hipLaunchKernelGGL((k_eam_density<1, true, SIM_EAM_TPW>),
                            dim3(persistent_grid((ntiles + SIM_EAM_TPW - 1) / SIM_EAM_TPW,
-                                                2 / SIM_EAM_TPW > 0 ? 2 / SIM_EAM_TPW : 1)),
+                                                eam_wg_per_cu(SIM_EAM_DENSITY_WG_PER_CU,
+                                                              SIM_EAM_TPW))),
                            dim3(SIM_TILE * SIM_EAM_TPW), tb, st, P, T, ntiles, X, nbr, nnbr,
                            n_stride, eam, rho, fp, W, partial, thr);
     else if (eam.ntypes == 1)
@@ -535,7 +552,8 @@ static int force_impl(const void* reps, const int* tiles, int ntiles, const int*
         if (use_lds_tab(eam, ntiles) && lds_ok)
             hipLaunchKernelGGL((k_eam_force<1, true, V, TPW, I>),
                                dim3(persistent_grid((ntiles + TPW - 1) / TPW,
-                                                    2 / TPW > 0 ? 2 / TPW : 1)),
+                                                    eam_wg_per_cu(SIM_EAM_FORCE_WG_PER_CU,
+                                                                  TPW))),
                                dim3(SIM_TILE * TPW), tb, st, P, T, ntiles, X, Q, nbr, nnbr,
                                n_stride, eam, force, partial, thr, S);
         else if (eam.ntypes == 1)
