@@ -22,6 +22,7 @@ learned MLP potential trained on EAM frames, run on 128,000 W atoms with EAM
 as the judge every 50 steps, against plain EAM).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -114,9 +115,13 @@ def kernel_time_ms(launch, reps=50):
 def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     """Roofline of the EAM passes with the algorithmic bytes per atom:
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
-    read) + 12 (F write). The dominant (longer) pass is reported; the kernels
-    timed are the ones the step launches (row kernels or global-list, with the
-    virial only when the step computes it: NPT)."""
+    read) + 12 (F write), and for the global list, whose force pass also does
+    the step's second half (sim_eam_force_second), + 12 (v_half read) + 12 (v
+    write). The dominant (longer) pass is reported; the kernels timed are the
+    ones the step launches (row kernels, or the global-list density pass and
+    fused force pass), with the virial as asked. Repeating the fused pass is
+    idempotent: v is rewritten from the same v_half, forces and noise (the
+    step counter does not advance)."""
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
@@ -137,20 +142,28 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     else:
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
+        st = eng.stage_arrays()
+        stp = ctypes.byref(st) if st is not None else None
 
         def p1():      # as the step launches it: records with F', no rho / F' arrays
-            call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
-                 ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, None, None,
-                 ptr(rec), ptr(eng.partial), stream_ptr())
+            if stp is None:
+                call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
+                     ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, None, None,
+                     ptr(rec), ptr(eng.partial), stream_ptr())
+            else:
+                call("sim_eam_density_staged", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
+                     ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, None, None,
+                     ptr(rec), ptr(eng.partial), stp, stream_ptr())
 
         def p2():
-            call("sim_eam_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-                 ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force),
-                 ptr(eng.partial), int(virial), stream_ptr())
+            call("sim_eam_force_second", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
+                 ptr(eng.posq), ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params,
+                 ptr(eng.force), ptr(eng.partial), int(virial), *eng.second_args(), stp,
+                 stream_ptr())
 
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
     b1 = eng.N * (12 + 4 * nbar + 4)
-    b2 = eng.N * (12 + 4 * nbar + 4 + 12)
+    b2 = eng.N * (12 + 4 * nbar + 4 + 12 + (24 if fam == "global" else 0))
     pre = "eam_force" + ("" if fam == "global" else "_" + fam)
     pre1 = "eam_density" + ("" if fam == "global" else "_" + fam)
     name, t, b = (pre, t2, b2) if t2 >= t1 else (pre1, t1, b1)
@@ -341,7 +354,7 @@ def bench_replicas(args):
     natoms = np.array([len(s.positions) for s in specs])
     log(f"replicas {len(specs)} atoms total {natoms.sum()} min {natoms.min()} max {natoms.max()}")
     eng = ReplicaEngine(specs, [lat.W_MASS], cutoff=5.5, skin=args.skin, seed=2024,
-                        family=args.family)
+                        family=args.family, stage=not args.no_stage)
     eng.force_field = EAMForceField(setfl, device=eng.device)
     eng.setup()
     # every step computes the virial tensor, as the reference's does
@@ -619,6 +632,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-stage", action="store_true",
+                    help="EAM passes read neighbour records from the list, not staged in LDS")
     ap.add_argument("--family", default=None, choices=["cell", "rows", "global"],
                     help="pair-kernel family of the batched engine (default: measured choice)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,

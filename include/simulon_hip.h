@@ -205,6 +205,37 @@ typedef struct sim_nl_reorder {
                               replicas (sim_pos_quantize), or NULL */
 } sim_nl_reorder_t;
 
+/* LDS staging of the EAM passes' neighbour records: no counterpart in the
+ * reference, whose kernels read every neighbour from global memory
+ * ("cuda source/eam_cuda_ext_kernel.cu:20-113"). A group is SIM_STAGE_TILES
+ * consecutive tiles (1,024 atoms); its members are the atoms of the cells
+ * in the 27-cell stencils of its atoms' cells, cell by cell. */
+#define SIM_STAGE_TILES 4
+#define SIM_STAGE_CAP_MAX 8192
+#define SIM_STAGE_TABLE 2048   /* cells of a group's stencil union at most */
+typedef struct sim_stage {
+    int* members;          /* [ngroups][cap] member atoms of each group */
+    int* count;            /* [ngroups] members of each group; -1: unstaged */
+    int* table;            /* [ngroups][3 * SIM_STAGE_TABLE] work space of sim_nl_stage */
+    unsigned short* slot;  /* [max_nbr][n] column-major like nbr: entry k of atom i as the
+                              position of that neighbour in its group's members; rows from
+                              the list's end to its wave's longest list (64 atoms of a
+                              tile) rounded up to 8 hold the atom's own position */
+    int cap;               /* member capacity per group, 1..SIM_STAGE_CAP_MAX */
+    int pad;
+} sim_stage_t;
+
+/* Members, counts and slots of every group that holds a tile of a replica
+ * with flag set (every group when flag is NULL), from the current lists and
+ * cells (run right after sim_nl_build with a reorder, same flag and arrays:
+ * the atoms of every cell contiguous, atom_cell[i] the cell of atom i);
+ * ngroups = ceil(ntiles / SIM_STAGE_TILES). A group whose members exceed
+ * cap gets count -1 and is read from the list by the kernels. */
+int sim_nl_stage(int R, const void* reps, const int* tiles, int ntiles, const int* flag,
+                 const int* nbr, const int* nnbr, int n, int max_nbr, const int* cell_rep,
+                 const int* atom_cell, const int* cell_start, const int* cell_count,
+                 const sim_stage_t* stage, void* stream);
+
 int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
                  double* pos4, double* pos_last, const int* flag,
                  const int* cell_rep, int ncells,
@@ -288,7 +319,25 @@ int sim_eam_force_second(int R, const void* reps, const int* tiles, int ntiles, 
                          sim_eam_params_t eam, float* force, double* partial, int virial,
                          sim_types_t types, const float* vel_half, float* vel, const float* noise,
                          const long long* key, unsigned long long seed,
-                         const unsigned long long* step_dev, void* stream);
+                         const unsigned long long* step_dev, const sim_stage_t* stage,
+                         void* stream);
+
+/* sim_eam_density and sim_eam_force with each group's member records staged
+ * in LDS (stage: sim_nl_stage of the current lists; sim_eam_force_second
+ * takes the same stage, or NULL). Every result equals the unstaged entry
+ * point's bit for bit: the same entries in the same order and masks, read
+ * from LDS instead of global memory. Used by the persistent one-type
+ * kernels (>= 256 tiles, table within the LDS limit), per group where its
+ * members fit in the LDS beside the table; the rest read the list. Replace
+ * the passes of core/force/eam_force_cu.py:461-579 like the unstaged ones. */
+int sim_eam_density_staged(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
+                           const int* nbr, const int* nnbr, int n_stride, sim_eam_params_t eam,
+                           float* rho, float* fp, int* rec, double* partial,
+                           const sim_stage_t* stage, void* stream);
+int sim_eam_force_staged(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
+                         const int* rec, const int* nbr, const int* nnbr, int n_stride,
+                         sim_eam_params_t eam, float* force, double* partial, int virial,
+                         const sim_stage_t* stage, void* stream);
 
 /* Allocation-free EAM evaluation: no counterpart in the reference. For
  * callers that cannot allocate device memory (code under HIP graph capture,

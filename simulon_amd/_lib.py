@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsimulon_
 SIM_TILE = 256
 SIM_MAX_TYPES = 4
 SIM_NPART = 16
+SIM_STAGE_TILES = 4
+SIM_STAGE_CAP_MAX = 8192
+SIM_STAGE_TABLE = 2048
 MLP_NDESC, MLP_NH, MLP_ROWS_PER_TILE = 32, 64, 4
 MLP_NPARAM = 6337
 MLP_NGRAD = MLP_NPARAM + 1
@@ -82,6 +85,12 @@ class NLReorder(ctypes.Structure):
                                    "row_base", "posq")]
 
 
+class Stage(ctypes.Structure):
+    """Mirror of sim_stage_t."""
+    _fields_ = [("members", _vp), ("count", _vp), ("table", _vp), ("slot", _vp),
+                ("cap", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
 # name -> (restype, argtypes)
 _I, _U64, _LL = ctypes.c_int, ctypes.c_ulonglong, ctypes.c_longlong
 SIGNATURES = {
@@ -114,7 +123,13 @@ SIGNATURES = {
     "sim_eam_energy_force_noalloc": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams,
                                           _vp, _vp, _I, _vp]),
     "sim_eam_force_second": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
-                                  _I, Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
+                                  _I, Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp, _vp]),
+    "sim_eam_density_staged": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
+                                    _vp, _vp, _vp, _vp]),
+    "sim_eam_force_staged": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
+                                  _I, _vp, _vp]),
+    "sim_nl_stage": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _vp,
+                          _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_first_q": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,

@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsimulon_hip.so")
-SOURCES = ["neighbor.hip", "pair_lj.hip", "pair_eam.hip", "pair_tiled.hip", "pair_rows.hip", "integrate.hip", "mlp.hip"]
+SOURCES = ["neighbor.hip", "pair_lj.hip", "pair_eam.hip", "pair_tiled.hip", "pair_rows.hip", "integrate.hip", "mlp.hip",
+           "stage.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

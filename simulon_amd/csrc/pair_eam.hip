@@ -74,29 +74,50 @@ __device__ __forceinline__ void tab_fill(float2* tl, const float4* __restrict__ 
     __syncthreads();
 }
 
-// Tile walk of one workgroup of TPW tile groups (256 threads each).
-// Persistent: the workgroups on XCD x (blockIdx.x % 8 == x, grid a multiple
-// of 8) take XCD x's contiguous run of tiles, TPW consecutive tiles per
-// workgroup and step; otherwise one tile per workgroup (xcd_block, TPW = 1).
-// Group g's tile is base + m + g while m + g < end.
-struct TileWalk {
-    int base, m, end, step;
-    __device__ __forceinline__ TileWalk(bool persistent, int ntiles, int tpw) {
+// Walk of one workgroup over groups of TPW consecutive tiles (256 threads
+// per tile; group G holds tiles G * TPW .. G * TPW + TPW - 1, the groups of
+// sim_nl_stage when TPW = SIM_STAGE_TILES). Persistent: the workgroups on
+// XCD x (blockIdx.x % 8 == x, grid a multiple of 8) take XCD x's contiguous
+// run of groups, one group per workgroup and step; otherwise one tile per
+// workgroup (xcd_block, TPW = 1).
+struct GroupWalk {
+    int G, end, step;
+    __device__ __forceinline__ GroupWalk(bool persistent, int ntiles, int tpw) {
         if (persistent) {
+            const int ng = (ntiles + tpw - 1) / tpw;
             const int x = blockIdx.x % SIM_NXCD;
-            const int q = ntiles / SIM_NXCD, r = ntiles % SIM_NXCD;
-            base = x * q + min(x, r);
-            end = q + (x < r ? 1 : 0);
-            m = (int)(blockIdx.x / SIM_NXCD) * tpw;
-            step = (int)(gridDim.x / SIM_NXCD) * tpw;
+            const int q = ng / SIM_NXCD, r = ng % SIM_NXCD;
+            const int base = x * q + min(x, r);
+            end = base + q + (x < r ? 1 : 0);
+            G = base + (int)(blockIdx.x / SIM_NXCD);
+            step = (int)(gridDim.x / SIM_NXCD);
         } else {
-            base = xcd_block(blockIdx.x, gridDim.x);
-            m = 0;
-            end = 1;
+            G = xcd_block(blockIdx.x, gridDim.x);
+            end = G + 1;
             step = 1;
         }
     }
 };
+
+// Entry k of atom i's slot list (sim_stage_t); k is wave-uniform and below
+// the list's padded rows.
+__device__ __forceinline__ int slot_row(const unsigned short* __restrict__ slot, int ns, int i,
+                                        int k) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b16(
+        buf_rsrc(slot + (size_t)k * ns, (unsigned)ns * 2u), i * 2, 0, 0);
+}
+
+// Padded slot rows of a wave whose longest list is cmax (at least 8, which
+// the list's row count always reaches).
+__device__ __forceinline__ int slot_rows(int cmax) { return max((cmax + 7) & ~7, 8); }
+
+// Copy the records of group G's members into LDS (all threads).
+__device__ __forceinline__ void stage_group(int4* S, const sim_stage_t& ST, int G, int nu,
+                                            const int4* __restrict__ src) {
+    const int* __restrict__ mem = ST.members + (size_t)G * ST.cap;
+    for (int q = threadIdx.x; q < nu; q += blockDim.x) S[q] = rec_at(src, mem[q]);
+    __syncthreads();
+}
 
 // Tile groups per workgroup of the persistent density pass: one table copy
 // in LDS serves them all, so more waves per CU share it.
@@ -135,10 +156,12 @@ constexpr int eam_wg_per_cu(int set, int tpw) { return set > 0 ? set : (2 / tpw 
 // One batch of UB list entries starting at k (indices in j[]). MASKED: the
 // batch may run past the end of the list; those entries are self (r = 0) and
 // masked out.
-template <int UB, bool MASKED, int NT, bool LT, bool ORTHO>
+// STAGED: j[] are member positions and the records come from the group's
+// LDS copy S; otherwise j[] are atoms and the records come from pos.
+template <int UB, bool MASKED, int NT, bool LT, bool ORTHO, bool STAGED = false>
 __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (&j)[UB], int k,
                                               int cnt, const int4 xi, int ti, const QBox& B,
-                                              const int4* __restrict__ pos,
+                                              const int4* __restrict__ pos, const int4* S,
                                               const sim_eam_params_t& E, const float2* tl,
                                               float thr) {
     const float4* __restrict__ T1 = static_cast<const float4*>(E.pass1);
@@ -148,7 +171,7 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
     Lerp L[UB];
     float4 v[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) xj[u] = rec_at(pos, j[u]);
+    for (int u = 0; u < UB; ++u) xj[u] = STAGED ? S[j[u]] : rec_at(pos, j[u]);
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         float rx, ry, rz;
@@ -195,26 +218,68 @@ __device__ __forceinline__ void density_atom(float& rho, float& ep, int i, const
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, k + U + u, cnt, cmax);
         if (k + U <= cmin)
-            density_batch<U, false, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
+            density_batch<U, false, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, nullptr, E,
+                                                   tl, thr);
         else
-            density_batch<U, true, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, E, tl, thr);
+            density_batch<U, true, NT, LT, ORTHO>(rho, ep, j, k, cnt, xi, ti, B, pos, nullptr, E,
+                                                  tl, thr);
     }
 }
 
-template <int NT, bool LT, int TPW>
+// The same walk over the slot list, records from the group's LDS copy (one
+// element type, LDS table): the same batches and masks, so the same sums.
+template <bool ORTHO>
+__device__ __forceinline__ void density_atom_staged(float& rho, float& ep, int i, const int4 xi,
+                                                    int cnt, int cmin, int cmax, const QBox& B,
+                                                    const int4* S,
+                                                    const unsigned short* __restrict__ slot,
+                                                    int ns, const sim_eam_params_t& E,
+                                                    const float2* tl, float thr) {
+    constexpr int U = SIM_EAM_LDS_BATCH;
+    static_assert(8 % U == 0, "batches must end within the padded slot rows");
+    const int rows = slot_rows(cmax);
+    int sn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) sn[u] = slot_row(slot, ns, i, min(u, rows - 1));
+    for (int k = 0; k < cmax; k += U) {
+        int sj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sj[u] = sn[u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sn[u] = slot_row(slot, ns, i, min(k + U + u, rows - 1));
+        if (k + U <= cmin)
+            density_batch<U, false, 1, true, ORTHO, true>(rho, ep, sj, k, cnt, xi, 0, B, nullptr,
+                                                          S, E, tl, thr);
+        else
+            density_batch<U, true, 1, true, ORTHO, true>(rho, ep, sj, k, cnt, xi, 0, B, nullptr, S,
+                                                         E, tl, thr);
+    }
+}
+
+// STG: a group's member records are staged in LDS (after the table) when
+// it has at most kcap members; other groups read the list.
+template <int NT, bool LT, int TPW, bool STG>
 __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1)
     k_eam_density(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                   int ntiles, const int4* __restrict__ pos, const int* __restrict__ nbr,
                   const int* __restrict__ nnbr, int n_stride, sim_eam_params_t E,
                   float* __restrict__ rho_out, float* __restrict__ fp_out, int4* __restrict__ rec,
-                  double* __restrict__ partial, float thr) {
+                  double* __restrict__ partial, float thr, sim_stage_t ST, int kcap) {
     extern __shared__ float2 tl[];
     __shared__ double lds[TPW * 4 * 1];
+    int4* SR = reinterpret_cast<int4*>(tl + ((E.n_r + 2) & ~1));
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass1), E.n_r);
-    for (TileWalk w(LT, ntiles, TPW); w.m < w.end; w.m += w.step) {
-        const bool have = w.m + g < w.end;
-        const int tb = w.base + w.m + (have ? g : 0);
+    for (GroupWalk w(LT, ntiles, TPW); w.G < w.end; w.G += w.step) {
+        const int tb0 = w.G * TPW;
+        const bool have = tb0 + g < ntiles;
+        const int tb = have ? tb0 + g : tb0;
+        bool staged = false;
+        if (STG) {
+            const int nu = ST.count[w.G];
+            staged = nu >= 0 && nu <= kcap;
+            if (staged) stage_group(SR, ST, w.G, nu, pos);
+        }
         const SimTile t = tiles[tb];
         const sim_replica_t& P = reps[t.rep];
         const int i = t.start + lt;
@@ -227,12 +292,20 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1
             const int4 xi = pos[i];
             const int ti = xi.w;
             float rho = 0.f, ep = 0.f;
-            if (P.ortho)
+            if (STG && staged) {
+                if (P.ortho)
+                    density_atom_staged<true>(rho, ep, i, xi, cnt, cmin, cmax, B, SR, ST.slot,
+                                              n_stride, E, tl, thr);
+                else
+                    density_atom_staged<false>(rho, ep, i, xi, cnt, cmin, cmax, B, SR, ST.slot,
+                                               n_stride, E, tl, thr);
+            } else if (P.ortho) {
                 density_atom<NT, LT, true>(rho, ep, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride, E,
                                            tl, thr);
-            else
+            } else {
                 density_atom<NT, LT, false>(rho, ep, i, xi, cnt, cmin, cmax, B, pos, nbr, n_stride,
                                             E, tl, thr);
+            }
             // embedding, eam_force_cu.py:448-459
             const float lo = E.rho_lo[ti];
             const float rcl = fminf(fmaxf(rho, lo), E.rho_hi[ti]);
@@ -251,7 +324,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1
         double v[1] = {etot};
         group_sum<1>(v, lds, g, lt);
         if (have && lt == 0) partial[(size_t)tb * SIM_NPART + SIM_T_EPOT] = v[0];
-        __syncthreads();   // lds is reused by the next tiles
+        __syncthreads();   // lds (and the staged records) are reused by the next group
     }
 }
 
@@ -285,10 +358,11 @@ __device__ __forceinline__ void eam_pair_force(ForceAcc& a, float rx, float ry, 
     }
 }
 
-template <int UB, bool MASKED, int NT, bool LT, bool VIRIAL, bool ORTHO>
+// STAGED as in density_batch.
+template <int UB, bool MASKED, int NT, bool LT, bool VIRIAL, bool ORTHO, bool STAGED = false>
 __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int k, int cnt,
                                             const int4 xi, int ti, float fpi, const QBox& B,
-                                            const int4* __restrict__ rec,
+                                            const int4* __restrict__ rec, const int4* S,
                                             const int4* __restrict__ typ,
                                             const sim_eam_params_t& E, const float2* tl,
                                             float thr) {
@@ -300,7 +374,7 @@ __device__ __forceinline__ void force_batch(ForceAcc& a, const int (&j)[UB], int
     Lerp L[UB];
     float4 va[UB], vb[UB];
 #pragma unroll
-    for (int u = 0; u < UB; ++u) xj[u] = rec_at(rec, j[u]);
+    for (int u = 0; u < UB; ++u) xj[u] = STAGED ? S[j[u]] : rec_at(rec, j[u]);
 #pragma unroll
     for (int u = 0; u < UB; ++u) tj[u] = NT == 1 ? 0 : rec_at(typ, j[u]).w;
 #pragma unroll
@@ -358,11 +432,42 @@ __device__ __forceinline__ void force_atom(ForceAcc& a, int i, const int4 xi, in
 #pragma unroll
         for (int u = 0; u < U; ++u) jn[u] = nbr_row(nbr, ns, i, k + U + u, cnt, cmax);
         if (k + U <= cmin)
-            force_batch<U, false, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
-                                                         tl, thr);
+            force_batch<U, false, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, nullptr,
+                                                         typ, E, tl, thr);
         else
-            force_batch<U, true, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, typ, E,
-                                                        tl, thr);
+            force_batch<U, true, NT, LT, VIRIAL, ORTHO>(a, j, k, cnt, xi, ti, fpi, B, rec, nullptr,
+                                                        typ, E, tl, thr);
+    }
+}
+
+// The same walk over the slot list, records from the group's LDS copy (one
+// element type, LDS table): the same batches and masks, so the same sums.
+template <bool VIRIAL, bool ORTHO>
+__device__ __forceinline__ void force_atom_staged(ForceAcc& a, int i, const int4 xi, int cnt,
+                                                  int cmin, int cmax, const QBox& B,
+                                                  const int4* S,
+                                                  const unsigned short* __restrict__ slot, int ns,
+                                                  const sim_eam_params_t& E, const float2* tl,
+                                                  float thr) {
+    constexpr int U = SIM_EAM_FORCE_LDS_BATCH;
+    static_assert(8 % U == 0, "batches must end within the padded slot rows");
+    const float fpi = __int_as_float(xi.w);
+    const int rows = slot_rows(cmax);
+    int sn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) sn[u] = slot_row(slot, ns, i, min(u, rows - 1));
+    for (int k = 0; k < cmax; k += U) {
+        int sj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sj[u] = sn[u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sn[u] = slot_row(slot, ns, i, min(k + U + u, rows - 1));
+        if (k + U <= cmin)
+            force_batch<U, false, 1, true, VIRIAL, ORTHO, true>(a, sj, k, cnt, xi, 0, fpi, B,
+                                                                nullptr, S, nullptr, E, tl, thr);
+        else
+            force_batch<U, true, 1, true, VIRIAL, ORTHO, true>(a, sj, k, cnt, xi, 0, fpi, B,
+                                                               nullptr, S, nullptr, E, tl, thr);
     }
 }
 
@@ -380,22 +485,30 @@ struct SecondHalf {
     sim_types_t ty;
 };
 
-template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG>
+template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG, bool STG>
 __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
                 int ntiles, const int4* __restrict__ typ, const int4* __restrict__ rec,
                 const int* __restrict__ nbr, const int* __restrict__ nnbr, int n_stride,
                 sim_eam_params_t E, float* __restrict__ force, double* __restrict__ partial,
-                float thr, SecondHalf S) {
+                float thr, SecondHalf S, sim_stage_t ST, int kcap) {
     extern __shared__ float2 tl[];
     constexpr int NV = INTEG ? 14 : 7;
     __shared__ double lds[TPW * 4 * NV];
+    int4* SR = reinterpret_cast<int4*>(tl + ((E.n_r + 2) & ~1));
     const unsigned long long step = (INTEG && S.step_dev) ? S.step_dev[0] : 0ull;
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass2), E.n_r);
-    for (TileWalk w(LT, ntiles, TPW); w.m < w.end; w.m += w.step) {
-        const bool have = w.m + g < w.end;
-        const int tb = w.base + w.m + (have ? g : 0);
+    for (GroupWalk w(LT, ntiles, TPW); w.G < w.end; w.G += w.step) {
+        const int tb0 = w.G * TPW;
+        const bool have = tb0 + g < ntiles;
+        const int tb = have ? tb0 + g : tb0;
+        bool staged = false;
+        if (STG) {
+            const int nu = ST.count[w.G];
+            staged = nu >= 0 && nu <= kcap;
+            if (staged) stage_group(SR, ST, w.G, nu, rec);
+        }
         const SimTile t = tiles[tb];
         const sim_replica_t& P = reps[t.rep];
         const int i = t.start + lt;
@@ -407,12 +520,20 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
             const QBox B = load_qbox(P);
             const int4 xi = rec[i];
             const int ti = NT == 1 ? 0 : typ[i].w;
-            if (P.ortho)
+            if (STG && staged) {
+                if (P.ortho)
+                    force_atom_staged<VIRIAL, true>(a, i, xi, cnt, cmin, cmax, B, SR, ST.slot,
+                                                    n_stride, E, tl, thr);
+                else
+                    force_atom_staged<VIRIAL, false>(a, i, xi, cnt, cmin, cmax, B, SR, ST.slot,
+                                                     n_stride, E, tl, thr);
+            } else if (P.ortho) {
                 force_atom<NT, LT, VIRIAL, true>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
                                                  n_stride, E, tl, thr);
-            else
+            } else {
                 force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
                                                   n_stride, E, tl, thr);
+            }
             force[(size_t)i * 3 + 0] = a.fx;
             force[(size_t)i * 3 + 1] = a.fy;
             force[(size_t)i * 3 + 2] = a.fz;
@@ -445,7 +566,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
                 for (int k = 0; k < 7; ++k) p[SIM_T_KE + k] = v[7 + k];
             }
         }
-        __syncthreads();   // lds is reused by the next tiles
+        __syncthreads();   // lds (and the staged records) are reused by the next group
     }
 }
 
@@ -489,17 +610,57 @@ bool allow_lds(K kernel) {
                                SIM_EAM_LDS_TAB_MAX) == hipSuccess;
 }
 
+// Dynamic LDS a staged kernel may take: the device's per-workgroup limit
+// less the kernel's static LDS, raised once per kernel; 0 if unavailable.
+template <typename K>
+int staged_lds(K kernel) {
+    int dev = 0, a = 0, b = 0;
+    hipFuncAttributes fa{};
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kernel)) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        a = 0;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess)
+        b = 0;
+    const int dyn = max(a, b) - (int)fa.sharedSizeBytes;
+    if (dyn <= 0 || hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        dyn) != hipSuccess)
+        return 0;
+    return dyn;
+}
+
+// Table bytes of the LDS copy rounded up to 16 (the staged records follow).
+size_t lds_tab_aligned(const sim_eam_params_t& E) { return (size_t)((E.n_r + 2) & ~1) * 8; }
+
+// Members per group the staged kernel holds in dyn bytes of LDS next to the
+// table (at most the stage's capacity); 0: staging off.
+int stage_kcap(const sim_stage_t* st, int dyn, const sim_eam_params_t& E) {
+    if (!st || dyn <= 0) return 0;
+    const long room = ((long)dyn - (long)lds_tab_aligned(E)) / 16;
+    return (int)max(0L, min(room, (long)st->cap));
+}
+
+bool stage_ok(const sim_stage_t* st) {
+    return st == nullptr ||
+           (st->members && st->count && st->slot && st->cap >= 1 && st->cap <= SIM_STAGE_CAP_MAX);
+}
+
+static_assert(SIM_EAM_TPW == SIM_STAGE_TILES && SIM_EAM_FORCE_TPW == SIM_STAGE_TILES,
+              "the persistent kernels walk the stage groups");
+
 }  // namespace
 
-extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
-                               const int* posq, const int* nbr, const int* nnbr, int n_stride,
-                               sim_eam_params_t eam, float* rho, float* fp, int* rec,
-                               double* partial, void* stream) {
-    (void)R;
+static int density_impl(const void* reps, const int* tiles, int ntiles, const int* posq,
+                        const int* nbr, const int* nnbr, int n_stride, sim_eam_params_t eam,
+                        float* rho, float* fp, int* rec, double* partial, const sim_stage_t* stg,
+                        void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
         n_stride >= (1 << 28) || (!fp && !rec) ||
         !partial || !eam.pass1 || !eam.emb || !eam.demb || eam.n_r < 2 || eam.n_rho < 2 ||
-        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (rec && (const int*)rec == posq))
+        eam.ntypes < 1 || eam.ntypes > SIM_MAX_TYPES || (rec && (const int*)rec == posq) ||
+        !stage_ok(stg))
         return SIM_E_ARG;
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
@@ -508,34 +669,64 @@ extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int nt
     auto* W = reinterpret_cast<int4*>(rec);
     hipStream_t st = as_stream(stream);
     const size_t tb = lds_tab_bytes(eam);
-    static const bool lds_ok = allow_lds(k_eam_density<1, true, SIM_EAM_TPW>);
-    if (use_lds_tab(eam, ntiles) && lds_ok)
-        hipLaunchKernelGGL((k_eam_density<1, true, SIM_EAM_TPW>),
-                           dim3(persistent_grid((ntiles + SIM_EAM_TPW - 1) / SIM_EAM_TPW,
-                                                eam_wg_per_cu(SIM_EAM_DENSITY_WG_PER_CU,
-                                                              SIM_EAM_TPW))),
-                           dim3(SIM_TILE * SIM_EAM_TPW), tb, st, P, T, ntiles, X, nbr, nnbr,
-                           n_stride, eam, rho, fp, W, partial, thr);
+    const sim_stage_t ST = stg ? *stg : sim_stage_t{};
+    constexpr int TPW = SIM_EAM_TPW;
+    const dim3 pg(persistent_grid((ntiles + TPW - 1) / TPW,
+                                  eam_wg_per_cu(SIM_EAM_DENSITY_WG_PER_CU, TPW)));
+    static const bool lds_ok = allow_lds(k_eam_density<1, true, TPW, false>);
+    static const int dyn_stg = staged_lds(k_eam_density<1, true, TPW, true>);
+    const int kcap = stage_kcap(stg, dyn_stg, eam);
+    if (use_lds_tab(eam, ntiles) && kcap > 0)
+        hipLaunchKernelGGL((k_eam_density<1, true, TPW, true>), pg, dim3(SIM_TILE * TPW),
+                           lds_tab_aligned(eam) + (size_t)kcap * 16, st, P, T, ntiles, X, nbr,
+                           nnbr, n_stride, eam, rho, fp, W, partial, thr, ST, kcap);
+    else if (use_lds_tab(eam, ntiles) && lds_ok)
+        hipLaunchKernelGGL((k_eam_density<1, true, TPW, false>), pg, dim3(SIM_TILE * TPW), tb, st,
+                           P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr, ST,
+                           0);
     else if (eam.ntypes == 1)
-        hipLaunchKernelGGL((k_eam_density<1, false, 1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P,
-                           T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<1, false, 1, false>), dim3(ntiles), dim3(SIM_TILE), 0,
+                           st, P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr,
+                           ST, 0);
     else
-        hipLaunchKernelGGL((k_eam_density<0, false, 1>), dim3(ntiles), dim3(SIM_TILE), 0, st, P,
-                           T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr);
+        hipLaunchKernelGGL((k_eam_density<0, false, 1, false>), dim3(ntiles), dim3(SIM_TILE), 0,
+                           st, P, T, ntiles, X, nbr, nnbr, n_stride, eam, rho, fp, W, partial, thr,
+                           ST, 0);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
+}
+
+extern "C" int sim_eam_density(int R, const void* reps, const int* tiles, int ntiles,
+                               const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                               sim_eam_params_t eam, float* rho, float* fp, int* rec,
+                               double* partial, void* stream) {
+    (void)R;
+    return density_impl(reps, tiles, ntiles, posq, nbr, nnbr, n_stride, eam, rho, fp, rec,
+                        partial, nullptr, stream);
+}
+
+extern "C" int sim_eam_density_staged(int R, const void* reps, const int* tiles, int ntiles,
+                                      const int* posq, const int* nbr, const int* nnbr,
+                                      int n_stride, sim_eam_params_t eam, float* rho, float* fp,
+                                      int* rec, double* partial, const sim_stage_t* stage,
+                                      void* stream) {
+    (void)R;
+    if (!stage) return SIM_E_ARG;
+    return density_impl(reps, tiles, ntiles, posq, nbr, nnbr, n_stride, eam, rho, fp, rec,
+                        partial, stage, stream);
 }
 
 static int force_impl(const void* reps, const int* tiles, int ntiles, const int* posq,
                       const int* rec, const int* nbr, const int* nnbr, int n_stride,
                       sim_eam_params_t eam, float* force, double* partial, int virial,
-                      const SecondHalf* S2, void* stream) {
+                      const SecondHalf* S2, const sim_stage_t* stg, void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !posq || !rec || !nbr || !nnbr || n_stride <= 0 ||
         n_stride >= (1 << 28) ||
         !force || !partial || !eam.pass2 || eam.n_r < 2 || eam.ntypes < 1 ||
-        eam.ntypes > SIM_MAX_TYPES)
+        eam.ntypes > SIM_MAX_TYPES || !stage_ok(stg))
         return SIM_E_ARG;
     const SecondHalf S = S2 ? *S2 : SecondHalf{};
+    const sim_stage_t ST = stg ? *stg : sim_stage_t{};
     const float thr = sqrt_threshold(eam.cutoff);
     auto* P = static_cast<const sim_replica_t*>(reps);
     auto* T = reinterpret_cast<const SimTile*>(tiles);
@@ -548,20 +739,25 @@ static int force_impl(const void* reps, const int* tiles, int ntiles, const int*
         constexpr bool V = decltype(v)::value;
         constexpr bool I = decltype(in)::value;
         constexpr int TPW = SIM_EAM_FORCE_TPW;
-        static const bool lds_ok = allow_lds(k_eam_force<1, true, V, TPW, I>);
-        if (use_lds_tab(eam, ntiles) && lds_ok)
-            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW, I>),
-                               dim3(persistent_grid((ntiles + TPW - 1) / TPW,
-                                                    eam_wg_per_cu(SIM_EAM_FORCE_WG_PER_CU,
-                                                                  TPW))),
-                               dim3(SIM_TILE * TPW), tb, st, P, T, ntiles, X, Q, nbr, nnbr,
-                               n_stride, eam, force, partial, thr, S);
+        const dim3 pg(persistent_grid((ntiles + TPW - 1) / TPW,
+                                      eam_wg_per_cu(SIM_EAM_FORCE_WG_PER_CU, TPW)));
+        static const bool lds_ok = allow_lds(k_eam_force<1, true, V, TPW, I, false>);
+        static const int dyn_stg = staged_lds(k_eam_force<1, true, V, TPW, I, true>);
+        const int kcap = stage_kcap(stg, dyn_stg, eam);
+        if (use_lds_tab(eam, ntiles) && kcap > 0)
+            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW, I, true>), pg, dim3(SIM_TILE * TPW),
+                               lds_tab_aligned(eam) + (size_t)kcap * 16, st, P, T, ntiles, X, Q,
+                               nbr, nnbr, n_stride, eam, force, partial, thr, S, ST, kcap);
+        else if (use_lds_tab(eam, ntiles) && lds_ok)
+            hipLaunchKernelGGL((k_eam_force<1, true, V, TPW, I, false>), pg, dim3(SIM_TILE * TPW),
+                               tb, st, P, T, ntiles, X, Q, nbr, nnbr, n_stride, eam, force,
+                               partial, thr, S, ST, 0);
         else if (eam.ntypes == 1)
-            hipLaunchKernelGGL((k_eam_force<1, false, V, 1, I>), g, b, 0, st, P, T, ntiles, X, Q,
-                               nbr, nnbr, n_stride, eam, force, partial, thr, S);
+            hipLaunchKernelGGL((k_eam_force<1, false, V, 1, I, false>), g, b, 0, st, P, T, ntiles,
+                               X, Q, nbr, nnbr, n_stride, eam, force, partial, thr, S, ST, 0);
         else
-            hipLaunchKernelGGL((k_eam_force<0, false, V, 1, I>), g, b, 0, st, P, T, ntiles, X, Q,
-                               nbr, nnbr, n_stride, eam, force, partial, thr, S);
+            hipLaunchKernelGGL((k_eam_force<0, false, V, 1, I, false>), g, b, 0, st, P, T, ntiles,
+                               X, Q, nbr, nnbr, n_stride, eam, force, partial, thr, S, ST, 0);
     };
     if (S2) {
         if (virial) go(std::true_type(), std::true_type());
@@ -580,7 +776,18 @@ extern "C" int sim_eam_force(int R, const void* reps, const int* tiles, int ntil
                              int virial, void* stream) {
     (void)R;
     return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
-                      virial, nullptr, stream);
+                      virial, nullptr, nullptr, stream);
+}
+
+extern "C" int sim_eam_force_staged(int R, const void* reps, const int* tiles, int ntiles,
+                                    const int* posq, const int* rec, const int* nbr,
+                                    const int* nnbr, int n_stride, sim_eam_params_t eam,
+                                    float* force, double* partial, int virial,
+                                    const sim_stage_t* stage, void* stream) {
+    (void)R;
+    if (!stage) return SIM_E_ARG;
+    return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
+                      virial, nullptr, stage, stream);
 }
 
 extern "C" int sim_eam_force_second(int R, const void* reps, const int* tiles, int ntiles,
@@ -589,12 +796,13 @@ extern "C" int sim_eam_force_second(int R, const void* reps, const int* tiles, i
                                     float* force, double* partial, int virial, sim_types_t types,
                                     const float* vel_half, float* vel, const float* noise,
                                     const long long* key, unsigned long long seed,
-                                    const unsigned long long* step_dev, void* stream) {
+                                    const unsigned long long* step_dev, const sim_stage_t* stage,
+                                    void* stream) {
     (void)R;
     if (!vel_half || !vel || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES) return SIM_E_ARG;
     const SecondHalf S{vel_half, vel, noise, key, seed, step_dev, types};
     return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
-                      virial, &S, stream);
+                      virial, &S, stage, stream);
 }
 
 // ---- allocation-free EAM evaluation ---------------------------------------

@@ -124,12 +124,14 @@ class ReplicaEngine:
 
     def __init__(self, specs, masses_amu, cutoff, skin, device="cuda", max_nbr=None,
                  precise_list=True, seed=12345, velocity_seed=None, sort=True, tiled=None,
-                 family=None):
+                 family=None, stage=True):
         """tiled / family choose the pair-kernel family: family "cell" (one
         workgroup per cell, 27-cell halo in LDS), "rows" (one workgroup per row
         of cells, sliding LDS window of stencil columns) or "global" (one
         thread per atom over the int32 list); tiled=True means "rows",
-        tiled=False "global"; both None = measured choice at setup()."""
+        tiled=False "global"; both None = measured choice at setup().
+        stage: let the global-list EAM passes stage each group's neighbour
+        records in LDS (stage_arrays); the same results bit for bit."""
         self.device = torch.device(device)
         if isinstance(specs, ReplicaSpec):
             specs = [specs]
@@ -143,6 +145,9 @@ class ReplicaEngine:
         self.precise_list = bool(precise_list)
         self._seed = int(seed)
         self.sort = bool(sort)
+        self.stage_on = bool(stage)
+        self.stage_cap = _lib.SIM_STAGE_CAP_MAX   # members per group (groups beyond: unstaged)
+        self._stage = None
         _lib.load()
         self._layout()
         self._alloc(max_nbr, velocity_seed)
@@ -521,6 +526,43 @@ class ReplicaEngine:
              ptr(self.atom_cell), ptr(self.atom_slot), ptr(self.cell_count), ptr(self.cell_start),
              ptr(self.cell_atoms), ptr(self.nbr), ptr(self.nnbr), self.max_nbr,
              ptr(self.overflow), int(self.precise_list), ctypes.byref(self._ro), s)
+        if self._stage is not None:
+            self._stage_build(self.flag)
+
+    # the persistent EAM kernels, which stage, run at >= 256 tiles
+    STAGE_MIN_TILES = 256
+
+    def stage_arrays(self):
+        """The sim_stage_t of this engine's lists (created, and built for every
+        group, on first use; from then on rebuilt with the lists, for the
+        groups of rebuilt replicas), or None when staging is off or the engine
+        is too small for the staged kernels."""
+        if not self.stage_on or self.ntiles < self.STAGE_MIN_TILES:
+            return None
+        if self._stage is None:
+            ng = -(-self.ntiles // _lib.SIM_STAGE_TILES)
+            cap = int(self.stage_cap)
+            if not 1 <= cap <= _lib.SIM_STAGE_CAP_MAX:
+                raise ValueError(f"stage_cap must be in 1..{_lib.SIM_STAGE_CAP_MAX}")
+            dev = self.device
+            self.stage_members = torch.zeros(ng * cap, dtype=torch.int32, device=dev)
+            self.stage_count = torch.full((ng,), -1, dtype=torch.int32, device=dev)
+            self.stage_slot = torch.zeros((self.max_nbr, self.N), dtype=torch.int16, device=dev)
+            self.stage_table = torch.zeros(ng * 3 * _lib.SIM_STAGE_TABLE, dtype=torch.int32,
+                                           device=dev)
+            st = _lib.Stage()
+            st.members, st.count = self.stage_members.data_ptr(), self.stage_count.data_ptr()
+            st.table = self.stage_table.data_ptr()
+            st.slot, st.cap = self.stage_slot.data_ptr(), cap
+            self._stage = st
+            self._stage_build(None)
+        return self._stage
+
+    def _stage_build(self, flag):
+        call("sim_nl_stage", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, ptr(flag),
+             ptr(self.nbr), ptr(self.nnbr), self.N, self.max_nbr, ptr(self.cell_rep),
+             ptr(self.atom_cell), ptr(self.cell_start), ptr(self.cell_count),
+             ctypes.byref(self._stage), stream_ptr())
 
     def quantize(self):
         """Fractional fixed-point records of the current positions (read by

@@ -1,5 +1,7 @@
 """Force fields as engine plug-ins: each writes forces and the energy/virial
 partial columns of every replica in one or two kernel launches."""
+import ctypes
+
 import numpy as np
 import torch
 
@@ -135,14 +137,22 @@ class EAMForceField:
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
         rho = eng.scratch("eam_rho", torch.float32) if keep_rho else None
-        call("sim_eam_density", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(rho), None, ptr(rec),
-             ptr(partial), s)
+        st = eng.stage_arrays() if hasattr(eng, "stage_arrays") else None
+        dargs = (eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq), ptr(eng.nbr),
+                 ptr(eng.nnbr), eng.N, self.params, ptr(rho), None, ptr(rec), ptr(partial))
+        if st is not None:
+            call("sim_eam_density_staged", *dargs, ctypes.byref(st), s)
+        else:
+            call("sim_eam_density", *dargs, s)
         args = (eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq), ptr(rec),
                 ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(force), ptr(partial),
                 int(bool(virial)))
         if second is not None and own:
-            call("sim_eam_force_second", *args, *second, s)
+            call("sim_eam_force_second", *args, *second,
+                 ctypes.byref(st) if st is not None else None, s)
             return True
-        call("sim_eam_force", *args, s)
+        if st is not None:
+            call("sim_eam_force_staged", *args, ctypes.byref(st), s)
+        else:
+            call("sim_eam_force", *args, s)
         return False
