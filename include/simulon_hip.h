@@ -38,7 +38,10 @@
  *        ("cuda source/ops.cpp:39-45") and core/force/lennard_jones_force.py:15-95
  *   sim_eam_density / sim_eam_force   simulon_cuda.density_pass / force_pass
  *        ("cuda source/eam_cuda_ext.cpp:30-60") and the rest of
- *        core/force/eam_force_cu.py:461-579
+ *        core/force/eam_force_cu.py:461-579; their _staged variants and
+ *        sim_eam_force_second (the force pass with the integrator's second
+ *        half) the same passes, with sim_nl_stage building the LDS staging
+ *        (no counterpart in the reference)
  *   sim_pos_quantize               Box.wrap_positions / minimum_image
  *        (core/box.py:109-142) in fractional fixed point
  *   sim_integrate_first / _second  core/integrator/integrator.py:91-124
