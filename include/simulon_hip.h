@@ -206,6 +206,10 @@ typedef struct sim_nl_reorder {
     int* row_base;         /* [R+1] prefix of the cell-row counts ny*nz, or NULL */
     int* posq;             /* [N][4] position records rewritten for the reordered
                               replicas (sim_pos_quantize), or NULL */
+    int stencil3_only;     /* 1: the caller guarantees >= 3 cells per dimension in every
+                              rebuilt replica (the thread-per-atom fallback list kernel,
+                              which serves the others, is not launched); 0: launch it */
+    int pad_ro;
 } sim_nl_reorder_t;
 
 /* LDS staging of the EAM passes' neighbour records: no counterpart in the

@@ -82,7 +82,8 @@ class NLReorder(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("vel", "force", "order", "slot_of", "image", "pos_tmp",
                                    "vel_tmp", "force_tmp", "order_tmp", "image_tmp", "cell_tmp",
                                    "nbr16", "stencil_max", "vel_half", "vel_half_tmp",
-                                   "row_base", "posq")]
+                                   "row_base", "posq")] + [("stencil3_only", ctypes.c_int),
+                                                           ("pad_ro", ctypes.c_int)]
 
 
 class Stage(ctypes.Structure):

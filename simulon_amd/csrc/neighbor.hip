@@ -775,10 +775,11 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
                                0, st, R, reps, atom_slot, x, cell_count, cell_start, nbr, nnbr,
                                n_stride, max_nbr, overflow, pos_last, ro->nbr16, ro->stencil_max);
         }
-        hipLaunchKernelGGL((k_list<TC, T, STRIDE, true>), dim3(ntiles), dim3(SIM_TILE), 0, st,
-                           reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,
-                           nbr, nnbr, n_stride, max_nbr, overflow, pos_last, ro->nbr16,
-                           ro->stencil_max, staged ? 1 : 0);
+        if (!(staged && ro->stencil3_only))
+            hipLaunchKernelGGL((k_list<TC, T, STRIDE, true>), dim3(ntiles), dim3(SIM_TILE), 0,
+                               st, reps, tiles, x, flag, atom_cell, cell_count, cell_start,
+                               cell_atoms, nbr, nnbr, n_stride, max_nbr, overflow, pos_last,
+                               ro->nbr16, ro->stencil_max, staged ? 1 : 0);
     } else {
         hipLaunchKernelGGL((k_list<TC, T, STRIDE, false>), dim3(ntiles), dim3(SIM_TILE), 0, st,
                            reps, tiles, x, flag, atom_cell, cell_count, cell_start, cell_atoms,

@@ -171,6 +171,10 @@ class ReplicaEngine:
         self._graph_virial = None
         self.steps_done = 0
         self.loading = False        # a tensile loader deforms the box at the start of each step
+        # True while no box has changed since construction (cleared by every
+        # write of a replica record, a deformation and a box assignment): the
+        # cell grids are then the ones can_tile was decided on
+        self.box_fixed = True
         self.profile = {"graph_replays": 0, "eager_steps": 0}
 
     # ---- layout --------------------------------------------------------------
@@ -519,6 +523,11 @@ class ReplicaEngine:
         self._ro.row_base = (self.row_base.data_ptr()
                              if force or (self.use_tiled and self.family == "rows") else None)
         self._ro.nbr16 = self.nbr16.data_ptr() if lds else None
+        # fixed boxes whose grids have >= 3 cells per dimension never need the
+        # thread-per-atom fallback list kernel (the device grid is the host's
+        # _grid_of while no box changes)
+        self._ro.stencil3_only = int(self.can_tile and self.box_fixed and not self.has_barostat
+                                     and not self.loading)
         call("sim_nl_trigger", self.R, ptr(self.reps), ptr(self.maxd2), ptr(self.flag),
              ptr(self.nbuild), int(bool(force)), s)
         call("sim_nl_build", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
@@ -676,6 +685,7 @@ class ReplicaEngine:
     def deform(self):
         """Scale the lattice rows of the replicas with loading = 1 by their
         deform factors, carrying the atoms with the box."""
+        self.box_fixed = False
         call("sim_box_deform", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
              ptr(self.pos4), ptr(self.pos_last), stream_ptr())
 
@@ -819,6 +829,7 @@ class ReplicaEngine:
         the same device memory, so the change applies to later replays."""
         import ctypes
         sz = ctypes.sizeof(_lib.Replica)
+        self.box_fixed = False
         raw = torch.frombuffer(bytearray(bytes(P)), dtype=torch.uint8)
         self.reps[r * sz:(r + 1) * sz].copy_(raw.to(self.device))
         self.specs[r].dt = P.dt

@@ -128,6 +128,7 @@ class AtomFileReader:
 
     def _box_changed(self, box):
         self.needs_update = True
+        self.engine.box_fixed = False
         self._graph_cache = None
 
     # ---- neighbor list ------------------------------------------------------------
