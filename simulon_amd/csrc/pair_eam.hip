@@ -185,12 +185,16 @@ __device__ __forceinline__ void density_batch(float& rho, float& ep, const int (
         v[u] = tab_get<LT>(T1, tl, (size_t)(ti * nt + tj) * E.n_r + L[u].idx);
     }
     // masked (not branched) so that the table loads of the batch stay
-    // unconditional and in flight together; + 0.0f leaves sums unchanged
+    // unconditional and in flight together: a 0/1 weight in one fused
+    // multiply-add per sum, fma(1, x, s) = s + x and fma(0, x, s) = s for the
+    // finite table values (a select of the term is turned into a branch per
+    // entry by the compiler)
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         const bool w = (!MASKED || k + u < cnt) && r2[u] < thr;
-        rho += w ? lerp2(v[u].x, v[u].y, L[u].frac) : 0.0f;
-        ep += w ? lerp2(v[u].z, v[u].w, L[u].frac) : 0.0f;
+        const float wf = w ? 1.0f : 0.0f;
+        rho = __builtin_fmaf(wf, lerp2(v[u].x, v[u].y, L[u].frac), rho);
+        ep = __builtin_fmaf(wf, lerp2(v[u].z, v[u].w, L[u].frac), ep);
     }
 }
 
