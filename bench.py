@@ -112,6 +112,20 @@ def kernel_time_ms(launch, reps=50):
     return a.elapsed_time(b) / reps
 
 
+def lockstep_rows(eng, counts):
+    """Mean over the waves (64 consecutive atoms of a tile) of the longest list
+    in the wave: the list rows a lock-step walk visits per atom."""
+    c = counts.cpu().numpy()
+    ends = np.array([eng.atom_base[r] + eng.natoms[r] for r in range(eng.R)])
+    rows = []
+    for r, s0 in eng.tiles_np:
+        for w0 in range(int(s0), min(int(s0) + 256, int(ends[r])), 64):
+            w1 = min(w0 + 64, int(ends[r]))
+            rows.append((int(c[w0:w1].max()), w1 - w0))
+    rows = np.array(rows)
+    return round(float((rows[:, 0] * rows[:, 1]).sum() / rows[:, 1].sum()), 2)
+
+
 def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     """Roofline of the EAM passes with the algorithmic bytes per atom:
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
@@ -162,6 +176,7 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
                  stream_ptr())
 
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
+    extra = {"lockstep_rows": lockstep_rows(eng, eng.nnbr)} if fam == "global" else {}
     b1 = eng.N * (12 + 4 * nbar + 4)
     b2 = eng.N * (12 + 4 * nbar + 4 + 12 + (24 if fam == "global" else 0))
     pre = "eam_force" + ("" if fam == "global" else "_" + fam)
@@ -173,7 +188,7 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
             "traffic": measured_traffic(workload, name),
             "traffic_ratio": traffic_ratio(workload, name),
             "bytes_per_launch": int(b), "avg_launch_ms": round(t, 5), "nbar_full": round(nbar, 2),
-            "other_pass_ms": round(t1 if t == t2 else t2, 5)}
+            "other_pass_ms": round(t1 if t == t2 else t2, 5), **extra}
 
 
 def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):

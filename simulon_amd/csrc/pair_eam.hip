@@ -76,26 +76,66 @@ __device__ __forceinline__ void tab_fill(float2* tl, const float4* __restrict__ 
 
 // Walk of one workgroup over groups of TPW consecutive tiles (256 threads
 // per tile; group G holds tiles G * TPW .. G * TPW + TPW - 1, the groups of
-// sim_nl_stage when TPW = SIM_STAGE_TILES). Persistent: the workgroups on
-// XCD x (blockIdx.x % 8 == x, grid a multiple of 8) take XCD x's contiguous
-// run of groups, one group per workgroup and step; otherwise one tile per
-// workgroup (xcd_block, TPW = 1).
+// sim_nl_stage when TPW = SIM_STAGE_TILES). Persistent: the groups are dealt
+// to the XCDs in chunks of SIM_EAM_XCD_CHUNK consecutive groups, chunk c to
+// XCD c % 8 (workgroup b runs on XCD b % 8, grid a multiple of 8), and an
+// XCD's workgroups take its groups in turn, one per workgroup and step.
+// Neighbouring groups share most of their neighbours, so a chunk keeps them
+// in one XCD's L2; dealing chunks round the XCDs spreads replicas of
+// different cost (list lengths grow with temperature, and the replicas are
+// packed in order) evenly over them. SIM_EAM_XCD_CHUNK 0: one contiguous run
+// of groups per XCD. Otherwise one tile per workgroup (xcd_block, TPW = 1).
+#ifndef SIM_EAM_XCD_CHUNK
+#define SIM_EAM_XCD_CHUNK 4
+#endif
 struct GroupWalk {
-    int G, end, step;
-    __device__ __forceinline__ GroupWalk(bool persistent, int ntiles, int tpw) {
+    int G, j, end, step, x, ng;
+    bool persistent;
+    __device__ __forceinline__ int group_of(int jj) const {
+        if (SIM_EAM_XCD_CHUNK == 0) return jj;
+        constexpr int C = SIM_EAM_XCD_CHUNK > 0 ? SIM_EAM_XCD_CHUNK : 1;
+        return (x + SIM_NXCD * (jj / C)) * C + jj % C;
+    }
+    __device__ __forceinline__ GroupWalk(bool pers, int ntiles, int tpw) : persistent(pers) {
         if (persistent) {
-            const int ng = (ntiles + tpw - 1) / tpw;
-            const int x = blockIdx.x % SIM_NXCD;
-            const int q = ng / SIM_NXCD, r = ng % SIM_NXCD;
-            const int base = x * q + min(x, r);
-            end = base + q + (x < r ? 1 : 0);
-            G = base + (int)(blockIdx.x / SIM_NXCD);
+            ng = (ntiles + tpw - 1) / tpw;
+            x = blockIdx.x % SIM_NXCD;
+            if (SIM_EAM_XCD_CHUNK == 0) {
+                const int q = ng / SIM_NXCD, r = ng % SIM_NXCD;
+                const int base = x * q + min(x, r);
+                j = base + (int)(blockIdx.x / SIM_NXCD);
+                end = base + q + (x < r ? 1 : 0);
+            } else {
+                // groups of XCD x: chunks x, x + 8, ...; the last may be short
+                constexpr int C = SIM_EAM_XCD_CHUNK > 0 ? SIM_EAM_XCD_CHUNK : 1;
+                const int nch = (ng + C - 1) / C;
+                const int mine = nch > x ? (nch - x + SIM_NXCD - 1) / SIM_NXCD : 0;
+                j = (int)(blockIdx.x / SIM_NXCD);
+                end = mine * C;   // indices j past ng are skipped in valid()
+            }
             step = (int)(gridDim.x / SIM_NXCD);
         } else {
-            G = xcd_block(blockIdx.x, gridDim.x);
-            end = G + 1;
+            x = 0;
+            ng = 0;
+            j = xcd_block(blockIdx.x, gridDim.x);
+            end = j + 1;
             step = 1;
         }
+        G = persistent ? group_of(j) : j;
+        skip();
+    }
+    // advance past indices whose group lies beyond the last (short last chunk)
+    __device__ __forceinline__ void skip() {
+        while (persistent && j < end && G >= ng) {
+            j += step;
+            G = group_of(j);
+        }
+    }
+    __device__ __forceinline__ bool valid() const { return j < end; }
+    __device__ __forceinline__ void next() {
+        j += step;
+        G = persistent ? group_of(j) : j;
+        skip();
     }
 };
 
@@ -111,11 +151,26 @@ __device__ __forceinline__ int slot_row(const unsigned short* __restrict__ slot,
 // the list's row count always reaches).
 __device__ __forceinline__ int slot_rows(int cmax) { return max((cmax + 7) & ~7, 8); }
 
-// Copy the records of group G's members into LDS (all threads).
+// Copy the records of group G's members into LDS (all threads), 8 members
+// per thread in flight: their member indices, then their records, are loaded
+// together, so a group costs about two memory round trips, not one per member
+// a thread copies.
 __device__ __forceinline__ void stage_group(int4* S, const sim_stage_t& ST, int G, int nu,
                                             const int4* __restrict__ src) {
+    constexpr int F = 8;
     const int* __restrict__ mem = ST.members + (size_t)G * ST.cap;
-    for (int q = threadIdx.x; q < nu; q += blockDim.x) S[q] = rec_at(src, mem[q]);
+    const int nt = (int)blockDim.x;
+    for (int q0 = threadIdx.x; q0 < nu; q0 += F * nt) {
+        int m[F];
+        int4 r[F];
+#pragma unroll
+        for (int u = 0; u < F; ++u) m[u] = mem[min(q0 + u * nt, nu - 1)];
+#pragma unroll
+        for (int u = 0; u < F; ++u) r[u] = rec_at(src, m[u]);
+#pragma unroll
+        for (int u = 0; u < F; ++u)
+            if (q0 + u * nt < nu) S[q0 + u * nt] = r[u];
+    }
     __syncthreads();
 }
 
@@ -274,7 +329,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1
     int4* SR = reinterpret_cast<int4*>(tl + ((E.n_r + 2) & ~1));
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass1), E.n_r);
-    for (GroupWalk w(LT, ntiles, TPW); w.G < w.end; w.G += w.step) {
+    for (GroupWalk w(LT, ntiles, TPW); w.valid(); w.next()) {
         const int tb0 = w.G * TPW;
         const bool have = tb0 + g < ntiles;
         const int tb = have ? tb0 + g : tb0;
@@ -503,7 +558,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     const unsigned long long step = (INTEG && S.step_dev) ? S.step_dev[0] : 0ull;
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     if (LT) tab_fill(tl, static_cast<const float4*>(E.pass2), E.n_r);
-    for (GroupWalk w(LT, ntiles, TPW); w.G < w.end; w.G += w.step) {
+    for (GroupWalk w(LT, ntiles, TPW); w.valid(); w.next()) {
         const int tb0 = w.G * TPW;
         const bool have = tb0 + g < ntiles;
         const int tb = have ? tb0 + g : tb0;

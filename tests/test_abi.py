@@ -73,20 +73,22 @@ def test_struct_layouts_match_c(tmp_path):
     assert got == want
 
 
-def test_replica_record_every_field_offset(tmp_path):
-    """Every field of sim_replica_t sits where the ctypes mirror puts it (the
-    device kernels and the host read the same bytes)."""
+@pytest.mark.parametrize("ctype, cname", [("Replica", "sim_replica_t"), ("Stage", "sim_stage_t")])
+def test_record_every_field_offset(tmp_path, ctype, cname):
+    """Every field of sim_replica_t and sim_stage_t sits where the ctypes
+    mirror puts it (the device kernels and the host read the same bytes)."""
     from simulon_amd import _lib
-    names = [f[0] for f in _lib.Replica._fields_]
-    body = "".join(f'printf("%zu\\n", offsetof(sim_replica_t, {n}));' for n in names)
+    cls = getattr(_lib, ctype)
+    names = [f[0] for f in cls._fields_]
+    body = "".join(f'printf("%zu\\n", offsetof({cname}, {n}));' for n in names)
     src = tmp_path / "offs.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "simulon_hip.h"\n'
-                   f'int main(){{{body}printf("%zu\\n", sizeof(sim_replica_t));return 0;}}\n')
+                   f'int main(){{{body}printf("%zu\\n", sizeof({cname}));return 0;}}\n')
     exe = tmp_path / "offs"
     r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                        capture_output=True, text=True)
     if r.returncode != 0:
         pytest.skip("no host C compiler: " + r.stderr)
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
-    want = [getattr(_lib.Replica, n).offset for n in names] + [ctypes.sizeof(_lib.Replica)]
+    want = [getattr(cls, n).offset for n in names] + [ctypes.sizeof(cls)]
     assert got == want
