@@ -575,6 +575,14 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
         ForceAcc a;
         const int cnt = active ? nnbr[i] : 0;
         const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
+        // the second half's O step needs no force: done before the walk, so
+        // its v_half load and random draw are off the group's critical path
+        float3 w_o = make_float3(0.f, 0.f, 0.f);
+        int ty_i = 0;
+        if (INTEG && active) {
+            ty_i = S.ty.ntypes == 1 ? 0 : typ[i].w;
+            w_o = second_half_pre(P, i, ty_i, S.vh, S.noise, S.key, S.seed, step);
+        }
         if (active) {
             const QBox B = load_qbox(P);
             const int4 xi = rec[i];
@@ -607,11 +615,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
         v[6] = 0.5 * a.wyz;
         if (INTEG) {
             double kin[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            if (active) {
-                const int ty_i = S.ty.ntypes == 1 ? 0 : typ[i].w;
-                second_half_atom(P, i, ty_i, S.ty, S.vh, a.fx, a.fy, a.fz, S.vel, S.noise, S.key,
-                                 S.seed, step, kin);
-            }
+            if (active) second_half_post(P, i, ty_i, S.ty, w_o, a.fx, a.fy, a.fz, S.vel, kin);
 #pragma unroll
             for (int k = 0; k < 7; ++k) v[7 + k] = kin[k];
         }

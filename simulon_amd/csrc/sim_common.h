@@ -176,14 +176,13 @@ __device__ __forceinline__ void gauss3(unsigned long long seed, unsigned long lo
 // k_second and the EAM force pass that fuses it, so both give the same bits.
 // noise: optional float[N][3] of N(0,1) draws (else Philox keyed by the
 // replica's key, the atom's index within its replica and the step).
-__device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, int ty_i,
-                                                 const sim_types_t& ty, const float* vh,
-                                                 float f0, float f1, float f2, float* vel,
-                                                 const float* noise, const long long* key,
-                                                 unsigned long long seed,
-                                                 unsigned long long step, double (&k)[7]) {
-    const float m = ty.mass[ty_i];
-    const float dt = P.dtf;
+// Split in two: second_half_pre (the O step: it needs no force, so the fused
+// force pass does it before its neighbour walk and keeps the result in
+// registers) and second_half_post (the B step, velocity and kinetic terms).
+__device__ __forceinline__ float3 second_half_pre(const sim_replica_t& P, int i, int ty_i,
+                                                  const float* vh, const float* noise,
+                                                  const long long* key, unsigned long long seed,
+                                                  unsigned long long step) {
     const size_t o = (size_t)i * 3;
     float v0 = vh[o], v1 = vh[o + 1], v2 = vh[o + 2];
     if (P.langevin) {
@@ -201,9 +200,18 @@ __device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, 
         v1 = v1 * P.damp + g1 * s;
         v2 = v2 * P.damp + g2 * s;
     }
-    v0 = v0 + (0.5f * (f0 / m)) * dt;
-    v1 = v1 + (0.5f * (f1 / m)) * dt;
-    v2 = v2 + (0.5f * (f2 / m)) * dt;
+    return make_float3(v0, v1, v2);
+}
+
+__device__ __forceinline__ void second_half_post(const sim_replica_t& P, int i, int ty_i,
+                                                 const sim_types_t& ty, float3 w, float f0,
+                                                 float f1, float f2, float* vel, double (&k)[7]) {
+    const float m = ty.mass[ty_i];
+    const float dt = P.dtf;
+    const size_t o = (size_t)i * 3;
+    const float v0 = w.x + (0.5f * (f0 / m)) * dt;
+    const float v1 = w.y + (0.5f * (f1 / m)) * dt;
+    const float v2 = w.z + (0.5f * (f2 / m)) * dt;
     vel[o] = v0;
     vel[o + 1] = v1;
     vel[o + 2] = v2;
@@ -215,4 +223,14 @@ __device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, 
     k[4] = md * a * b;
     k[5] = md * a * c;
     k[6] = md * b * c;
+}
+
+__device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, int ty_i,
+                                                 const sim_types_t& ty, const float* vh,
+                                                 float f0, float f1, float f2, float* vel,
+                                                 const float* noise, const long long* key,
+                                                 unsigned long long seed,
+                                                 unsigned long long step, double (&k)[7]) {
+    const float3 w = second_half_pre(P, i, ty_i, vh, noise, key, seed, step);
+    second_half_post(P, i, ty_i, ty, w, f0, f1, f2, vel, k);
 }
