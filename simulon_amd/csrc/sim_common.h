@@ -46,10 +46,33 @@ struct SimTile {
     int start;
 };
 
+// Lane l' of v within its row of 16 lanes by a DPP lane pattern (ctrl), for
+// both 32-bit halves of a double (VALU only: no LDS crossbar traffic).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes (all active), the same value in every lane: pairs,
+// quads, half rows and rows of 16 by DPP (each step adds two equal-order
+// partial sums, so all lanes of a row agree bit for bit), then the four row
+// sums in a fixed order. Deterministic; no LDS traffic.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_d<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_d<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_d<0x141>(v);   // row_half_mirror: the other quad of the half row
+    v += dpp_d<0x140>(v);   // row_mirror: the other half row
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
