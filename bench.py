@@ -131,7 +131,8 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
     pass 1: 12 (x) + 4 nbar (list) + 4 (F' write); pass 2: 12 + 4 nbar + 4 (F'
     read) + 12 (F write), and for the global list, whose force pass also does
     the step's second half (sim_eam_force_second), + 12 (v_half read) + 12 (v
-    write). The dominant (longer) pass is reported; the kernels timed are the
+    write), + 124 when graph replays let it do the next step's first half
+    too. The dominant (longer) pass is reported; the kernels timed are the
     ones the step launches (row kernels, or the global-list density pass and
     fused force pass), with the virial as asked. Repeating the fused pass is
     idempotent: v is rewritten from the same v_half, forces and noise (the
@@ -169,16 +170,34 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
                      ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, None, None,
                      ptr(rec), ptr(eng.partial), stp, stream_ptr())
 
+        # graph replays of two or more steps launch the variant that also does
+        # the next step's first half: timed on copies of the positions,
+        # records and displacement maxima, with v_half restored afterwards
+        fuse = eng.fuses_next_first()
+        if fuse:
+            nxt_pos, nxt_q = eng.pos4.clone(), eng.posq.clone()
+            nxt_d2 = torch.zeros_like(eng.maxd2)
+            nxt = (ptr(nxt_pos), ptr(eng.pos_last), ptr(nxt_d2), ptr(nxt_q))
+            saved = (eng.vel_half.clone(), eng.vel.clone())
+        else:
+            nxt = (None, None, None, None)
+
         def p2():
             call("sim_eam_force_second", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
                  ptr(eng.posq), ptr(rec), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params,
-                 ptr(eng.force), ptr(eng.partial), int(virial), *eng.second_args(), stp,
+                 ptr(eng.force), ptr(eng.partial), int(virial), *eng.second_args(), stp, *nxt,
                  stream_ptr())
 
     t1, t2 = kernel_time_ms(p1), kernel_time_ms(p2)
+    if fam == "global" and fuse:
+        eng.vel_half.copy_(saved[0])
+        eng.vel.copy_(saved[1])
     extra = {"lockstep_rows": lockstep_rows(eng, eng.nnbr)} if fam == "global" else {}
     b1 = eng.N * (12 + 4 * nbar + 4)
-    b2 = eng.N * (12 + 4 * nbar + 4 + 12 + (24 if fam == "global" else 0))
+    # + 124 with the next first half: x and the build position read (2 x 32),
+    # v_half, x and the record written (12 + 32 + 16)
+    b2 = eng.N * (12 + 4 * nbar + 4 + 12 + (24 if fam == "global" else 0)
+                  + (124 if fam == "global" and fuse else 0))
     pre = "eam_force" + ("" if fam == "global" else "_" + fam)
     pre1 = "eam_density" + ("" if fam == "global" else "_" + fam)
     name, t, b = (pre, t2, b2) if t2 >= t1 else (pre1, t1, b1)

@@ -321,13 +321,21 @@ int sim_eam_force(int R, const void* reps, const int* tiles, int ntiles,
  * SIM_T_KE..SIM_T_KE+6 of partial, so the step needs no separate
  * second-half kernel. Replaces core/integrator/integrator.py:102-124 after
  * the EAM force pass of core/force/eam_force_cu.py:555-570. */
+/* next_pos4 (optional, one element type; no counterpart in the reference):
+ * for the replicas with neither a barostat nor loading, the pass then also
+ * does the next step's first half (sim_integrate_first_q's arithmetic, bit
+ * for bit): vel_half, next_pos4 and the records next_posq (the posq of this
+ * call) of the new positions, and the displacement maxima since the build
+ * (pos_last) into maxd2; that step then calls sim_integrate_first_q with
+ * unfused_only = 1. */
 int sim_eam_force_second(int R, const void* reps, const int* tiles, int ntiles, const int* posq,
                          const int* rec, const int* nbr, const int* nnbr, int n_stride,
                          sim_eam_params_t eam, float* force, double* partial, int virial,
-                         sim_types_t types, const float* vel_half, float* vel, const float* noise,
+                         sim_types_t types, float* vel_half, float* vel, const float* noise,
                          const long long* key, unsigned long long seed,
                          const unsigned long long* step_dev, const sim_stage_t* stage,
-                         void* stream);
+                         double* next_pos4, const double* pos_last, double* maxd2,
+                         int* next_posq, void* stream);
 
 /* sim_eam_density and sim_eam_force with each group's member records staged
  * in LDS (stage: sim_nl_stage of the current lists; sim_eam_force_second
@@ -425,11 +433,13 @@ int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
 
 /* sim_integrate_first that also writes the position records of the new
  * positions (as sim_pos_quantize) into posq [N][4]: a step whose list build
- * passes the same posq in sim_nl_reorder_t needs no sim_pos_quantize. */
+ * passes the same posq in sim_nl_reorder_t needs no sim_pos_quantize.
+ * unfused_only = 1: only the replicas with a barostat or loading (the others'
+ * first half was done by the previous step's sim_eam_force_second). */
 int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
                           sim_types_t types, double* pos4, const float* vel,
                           const float* force, float* vel_half, const double* pos_last,
-                          double* maxd2, int* posq, void* stream);
+                          double* maxd2, int* posq, int unfused_only, void* stream);
 
 /* Second half (O + B): v = vh damp + xi rff (Langevin replicas), then
  * v += 1/2 F/m dt; KE and kinetic-tensor columns of partial.

@@ -124,7 +124,8 @@ SIGNATURES = {
     "sim_eam_energy_force_noalloc": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams,
                                           _vp, _vp, _I, _vp]),
     "sim_eam_force_second": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
-                                  _I, Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp, _vp]),
+                                  _I, Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp, _vp, _vp, _vp,
+                                  _vp, _vp]),
     "sim_eam_density_staged": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
                                     _vp, _vp, _vp, _vp]),
     "sim_eam_force_staged": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _I, EAMParams, _vp, _vp,
@@ -132,7 +133,8 @@ SIGNATURES = {
     "sim_nl_stage": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _vp,
                           _vp]),
     "sim_integrate_first": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "sim_integrate_first_q": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sim_integrate_first_q": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _I,
+                                   _vp]),
     "sim_integrate_second": (_I, [_vp, _vp, _I, Types, _vp, _vp, _vp, _vp, _vp, _vp, _U64, _U64, _vp,
                                   _vp, _vp]),
     "sim_step_advance": (_I, [_vp, _vp]),

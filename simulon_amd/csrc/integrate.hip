@@ -21,31 +21,26 @@ __global__ void __launch_bounds__(SIM_TILE)
             sim_types_t ty, double4* __restrict__ pos, const float* __restrict__ vel,
             const float* __restrict__ force, float* __restrict__ vh_out,
             const double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2,
-            int4* __restrict__ posq) {
+            int4* __restrict__ posq, int unfused_only) {
     __shared__ double lds[4];
     const SimTile t = tile_of(tiles);
     const sim_replica_t& P = reps[t.rep];
+    // the previous step's EAM force pass did this replica's first half
+    if (unfused_only && first_fusable(P)) return;   // uniform over the tile
     const int i = t.start + threadIdx.x;
     double d2 = 0.0;
     if (i < P.atom_base + P.natoms) {
         double4 x = pos[i];
         const float m = ty.mass[(int)x.w];
-        const float dt = P.dtf;
         const size_t o = (size_t)i * 3;
-        const float v0 = vel[o] + (0.5f * (force[o] / m)) * dt;
-        const float v1 = vel[o + 1] + (0.5f * (force[o + 1] / m)) * dt;
-        const float v2 = vel[o + 2] + (0.5f * (force[o + 2] / m)) * dt;
-        vh_out[o] = v0;
-        vh_out[o + 1] = v1;
-        vh_out[o + 2] = v2;
-        x.x += (double)v0 * P.dt;
-        x.y += (double)v1 * P.dt;
-        x.z += (double)v2 * P.dt;
+        float3 h;
+        d2 = first_half_atom(P, m, make_float3(vel[o], vel[o + 1], vel[o + 2]), force[o],
+                             force[o + 1], force[o + 2], x, pos_last[i], h);
+        vh_out[o] = h.x;
+        vh_out[o + 1] = h.y;
+        vh_out[o + 2] = h.z;
         pos[i] = x;
         if (posq) posq[i] = quant_record(P, x);   // records for this step's forces
-        const double4 l = pos_last[i];
-        const double ex = x.x - l.x, ey = x.y - l.y, ez = x.z - l.z;
-        d2 = (ex * ex + ey * ey) + ez * ez;
     }
     d2 = wave_max(d2);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = d2;
@@ -446,7 +441,8 @@ extern "C" int sim_pos_quantize(const void* reps, const int* tiles, int ntiles, 
 
 static int first_impl(const void* reps, const int* tiles, int ntiles, sim_types_t types,
                       double* pos4, const float* vel, const float* force, float* vel_half,
-                      const double* pos_last, double* maxd2, int* posq, void* stream) {
+                      const double* pos_last, double* maxd2, int* posq, int unfused_only,
+                      void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !pos4 || !vel || !force || !vel_half || !pos_last ||
         !maxd2 || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES)
         return SIM_E_ARG;
@@ -455,7 +451,8 @@ static int first_impl(const void* reps, const int* tiles, int ntiles, sim_types_
                        reinterpret_cast<const SimTile*>(tiles), types,
                        reinterpret_cast<double4*>(pos4), vel, force, vel_half,
                        reinterpret_cast<const double4*>(pos_last),
-                       reinterpret_cast<unsigned long long*>(maxd2), reinterpret_cast<int4*>(posq));
+                       reinterpret_cast<unsigned long long*>(maxd2), reinterpret_cast<int4*>(posq),
+                       unfused_only);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
@@ -465,16 +462,17 @@ extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntile
                                    const float* force, float* vel_half, const double* pos_last,
                                    double* maxd2, void* stream) {
     return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
-                      nullptr, stream);
+                      nullptr, 0, stream);
 }
 
 extern "C" int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
                                      sim_types_t types, double* pos4, const float* vel,
                                      const float* force, float* vel_half, const double* pos_last,
-                                     double* maxd2, int* posq, void* stream) {
+                                     double* maxd2, int* posq, int unfused_only,
+                                     void* stream) {
     if (!posq) return SIM_E_ARG;
     return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
-                      posq, stream);
+                      posq, unfused_only, stream);
 }
 
 extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntiles,

@@ -117,6 +117,12 @@ __device__ __forceinline__ void group_sum(double (&v)[NV], double* lds, int g, i
     }
 }
 
+// An int the caller knows to be equal on every lane of the wave (such as a
+// tile index derived from threadIdx.x / 256), moved to a scalar register: the
+// tile record, the replica record and every field read through them then
+// come by scalar loads instead of one vector load per lane.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Wave-uniform extrema of an int over the 64 lanes (all lanes active): DPP
 // within rows of 16, then the four row results (VALU only, as wave_sum).
 template <int CTRL>

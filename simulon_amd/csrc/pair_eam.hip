@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_DENSITY_WAVES : 1
     for (GroupWalk w(LT, ntiles, TPW); w.valid(); w.next()) {
         const int tb0 = w.G * TPW;
         const bool have = tb0 + g < ntiles;
-        const int tb = have ? tb0 + g : tb0;
+        const int tb = wave_uniform(have ? tb0 + g : tb0);   // g = threadIdx.x / 256
         bool staged = false;
         if (STG) {
             const int nu = ST.count[w.G];
@@ -534,14 +534,21 @@ __device__ __forceinline__ void force_atom_staged(ForceAcc& a, int i, const int4
 // force of atom i in registers the pass finishes the step for it (O + B, the
 // arithmetic of k_second) and writes the kinetic columns of its tile's
 // partial row, instead of a separate kernel re-reading forces and v_half.
+// With pos set, the pass also does the next step's first half for the
+// replicas first_fusable admits (force and new velocity in registers):
+// v_half, position, record, and the displacement maximum for the trigger.
 struct SecondHalf {
-    const float* vh;
+    float* vh;
     float* vel;
     const float* noise;
     const long long* key;
     unsigned long long seed;
     const unsigned long long* step_dev;
     sim_types_t ty;
+    double4* pos;
+    const double4* pos_last;
+    unsigned long long* maxd2;
+    int4* posq;
 };
 
 template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG, bool STG>
@@ -554,6 +561,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     extern __shared__ float2 tl[];
     constexpr int NV = INTEG ? 14 : 7;
     __shared__ double lds[TPW * 4 * NV];
+    __shared__ double wmax[INTEG ? TPW * 4 : 1];
     int4* SR = reinterpret_cast<int4*>(tl + ((E.n_r + 2) & ~1));
     const unsigned long long step = (INTEG && S.step_dev) ? S.step_dev[0] : 0ull;
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
@@ -561,7 +569,7 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     for (GroupWalk w(LT, ntiles, TPW); w.valid(); w.next()) {
         const int tb0 = w.G * TPW;
         const bool have = tb0 + g < ntiles;
-        const int tb = have ? tb0 + g : tb0;
+        const int tb = wave_uniform(have ? tb0 + g : tb0);   // g = threadIdx.x / 256
         bool staged = false;
         if (STG) {
             const int nu = ST.count[w.G];
@@ -573,6 +581,8 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
         const int i = t.start + lt;
         const bool active = have && i < P.atom_base + P.natoms;
         ForceAcc a;
+        const bool next = INTEG && S.pos != nullptr && first_fusable(P);   // tile-uniform
+        double4 xn = make_double4(0.0, 0.0, 0.0, 0.0), ln = xn;
         const int cnt = active ? nnbr[i] : 0;
         const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
         // the second half's O step needs no force: done before the walk, so
@@ -601,6 +611,10 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
                 force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
                                                   n_stride, E, tl, thr);
             }
+            if (next) {   // issued before the stores below, so their latency overlaps them
+                xn = S.pos[i];
+                ln = S.pos_last[i];
+            }
             force[(size_t)i * 3 + 0] = a.fx;
             force[(size_t)i * 3 + 1] = a.fy;
             force[(size_t)i * 3 + 2] = a.fz;
@@ -615,11 +629,31 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
         v[6] = 0.5 * a.wyz;
         if (INTEG) {
             double kin[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            if (active) second_half_post(P, i, ty_i, S.ty, w_o, a.fx, a.fy, a.fz, S.vel, kin);
+            double d2 = 0.0;
+            // one type: a constant kernel-argument offset, so a scalar load
+            const float m_i = S.ty.ntypes == 1 ? S.ty.mass[0] : S.ty.mass[ty_i];
+            if (active) {
+                const float3 vn =
+                    second_half_post(P, i, m_i, w_o, a.fx, a.fy, a.fz, S.vel, kin);
+                if (next) {   // the next step's first half (k_first's arithmetic)
+                    float3 h;
+                    d2 = first_half_atom(P, m_i, vn, a.fx, a.fy, a.fz, xn, ln, h);
+                    const size_t o = (size_t)i * 3;
+                    S.vh[o] = h.x;   // this atom's v_half, read by its own thread above
+                    S.vh[o + 1] = h.y;
+                    S.vh[o + 2] = h.z;
+                    S.pos[i] = xn;
+                    S.posq[i] = quant_record(P, xn);
+                }
+            }
+            if (next) {
+                d2 = wave_max(d2);
+                if ((lt & 63) == 0) wmax[g * 4 + (lt >> 6)] = d2;
+            }
 #pragma unroll
             for (int k = 0; k < 7; ++k) v[7 + k] = kin[k];
         }
-        group_sum<NV>(v, lds, g, lt);
+        group_sum<NV>(v, lds, g, lt);   // its barrier also publishes wmax
         if (have && lt == 0) {
             double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
@@ -627,6 +661,14 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
             if (INTEG) {
 #pragma unroll
                 for (int k = 0; k < 7; ++k) p[SIM_T_KE + k] = v[7 + k];
+            }
+            if (next) {
+                const double* q = wmax + g * 4;
+                const double m = fmax(fmax(q[0], q[1]), fmax(q[2], q[3]));
+                // non-negative doubles order like their bit patterns; the plain
+                // read first skips most atomics (as k_first)
+                const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+                if (b > S.maxd2[t.rep]) atomicMax(&S.maxd2[t.rep], b);
             }
         }
         __syncthreads();   // lds (and the staged records) are reused by the next group
@@ -857,13 +899,28 @@ extern "C" int sim_eam_force_second(int R, const void* reps, const int* tiles, i
                                     const int* posq, const int* rec, const int* nbr,
                                     const int* nnbr, int n_stride, sim_eam_params_t eam,
                                     float* force, double* partial, int virial, sim_types_t types,
-                                    const float* vel_half, float* vel, const float* noise,
+                                    float* vel_half, float* vel, const float* noise,
                                     const long long* key, unsigned long long seed,
                                     const unsigned long long* step_dev, const sim_stage_t* stage,
-                                    void* stream) {
+                                    double* next_pos4, const double* pos_last, double* maxd2,
+                                    int* next_posq, void* stream) {
     (void)R;
     if (!vel_half || !vel || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES) return SIM_E_ARG;
-    const SecondHalf S{vel_half, vel, noise, key, seed, step_dev, types};
+    // the next step's first half: one element type (the pass then never reads
+    // the records it rewrites)
+    if (next_pos4 && (!pos_last || !maxd2 || !next_posq || eam.ntypes != 1 || types.ntypes != 1))
+        return SIM_E_ARG;
+    const SecondHalf S{vel_half,
+                       vel,
+                       noise,
+                       key,
+                       seed,
+                       step_dev,
+                       types,
+                       reinterpret_cast<double4*>(next_pos4),
+                       reinterpret_cast<const double4*>(pos_last),
+                       reinterpret_cast<unsigned long long*>(maxd2),
+                       reinterpret_cast<int4*>(next_posq)};
     return force_impl(reps, tiles, ntiles, posq, rec, nbr, nnbr, n_stride, eam, force, partial,
                       virial, &S, stage, stream);
 }

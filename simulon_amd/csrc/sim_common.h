@@ -226,10 +226,10 @@ __device__ __forceinline__ float3 second_half_pre(const sim_replica_t& P, int i,
     return make_float3(v0, v1, v2);
 }
 
-__device__ __forceinline__ void second_half_post(const sim_replica_t& P, int i, int ty_i,
-                                                 const sim_types_t& ty, float3 w, float f0,
-                                                 float f1, float f2, float* vel, double (&k)[7]) {
-    const float m = ty.mass[ty_i];
+// m: the atom's mass. Returns the new velocity (the values written to vel).
+__device__ __forceinline__ float3 second_half_post(const sim_replica_t& P, int i, float m,
+                                                   float3 w, float f0, float f1, float f2,
+                                                   float* vel, double (&k)[7]) {
     const float dt = P.dtf;
     const size_t o = (size_t)i * 3;
     const float v0 = w.x + (0.5f * (f0 / m)) * dt;
@@ -246,6 +246,7 @@ __device__ __forceinline__ void second_half_post(const sim_replica_t& P, int i, 
     k[4] = md * a * b;
     k[5] = md * a * c;
     k[6] = md * b * c;
+    return make_float3(v0, v1, v2);
 }
 
 __device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, int ty_i,
@@ -255,5 +256,32 @@ __device__ __forceinline__ void second_half_atom(const sim_replica_t& P, int i, 
                                                  unsigned long long seed,
                                                  unsigned long long step, double (&k)[7]) {
     const float3 w = second_half_pre(P, i, ty_i, vh, noise, key, seed, step);
-    second_half_post(P, i, ty_i, ty, w, f0, f1, f2, vel, k);
+    (void)second_half_post(P, i, ty.mass[ty_i], w, f0, f1, f2, vel, k);
+}
+
+// First half of the velocity-Verlet step (core/integrator/integrator.py:91-99)
+// for one atom of mass m (float32 velocities with the reference's
+// arithmetic, float64 positions): h = v + (0.5 F/m) dt, x += double(h) dt.
+// Returns |x - l|^2, the displacement since the last list build. Shared by
+// k_first and the EAM force pass that does the next step's first half, so
+// both give the same bits.
+__device__ __forceinline__ double first_half_atom(const sim_replica_t& P, float m, float3 v,
+                                                  float f0, float f1, float f2, double4& x,
+                                                  const double4& l, float3& h) {
+    const float dt = P.dtf;
+    h.x = v.x + (0.5f * (f0 / m)) * dt;
+    h.y = v.y + (0.5f * (f1 / m)) * dt;
+    h.z = v.z + (0.5f * (f2 / m)) * dt;
+    x.x += (double)h.x * P.dt;
+    x.y += (double)h.y * P.dt;
+    x.z += (double)h.z * P.dt;
+    const double ex = x.x - l.x, ey = x.y - l.y, ez = x.z - l.z;
+    return (ex * ex + ey * ey) + ez * ez;
+}
+
+// Replicas whose next first half the fused EAM force pass may do: no
+// barostat and no loading step between this step's end and the next step's
+// first half.
+__device__ __forceinline__ bool first_fusable(const sim_replica_t& P) {
+    return P.barostat == 0 && P.loading == 0;
 }

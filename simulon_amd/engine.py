@@ -169,6 +169,7 @@ class ReplicaEngine:
         self._scratch = {}
         self._graph = None
         self._graph_virial = None
+        self._fuse_next = False
         self.steps_done = 0
         self.loading = False        # a tensile loader deforms the box at the start of each step
         # True while no box has changed since construction (cleared by every
@@ -598,6 +599,9 @@ class ReplicaEngine:
                 self.compute_forces(virial=virial)
         finally:
             self._posq_fresh = False
+        if self._fuse_next and not fused:
+            raise RuntimeError("the next step's first half was to be fused into a force pass "
+                               "that does not fuse the second half")
         if second and not fused:
             self.second_half(noise, advance=False)
 
@@ -607,19 +611,37 @@ class ReplicaEngine:
         return (self.types_struct, ptr(self.vel_half), ptr(self.vel), ptr(noise), ptr(self.order),
                 0, ptr(self.step_ctr))
 
+    def next_first_args(self):
+        """Arguments of sim_eam_force_second for the next step's first half
+        (positions, build positions, displacement maxima, records), or four
+        NULLs when the step being recorded does not fuse it."""
+        if not self._fuse_next:
+            return (None, None, None, None)
+        return (ptr(self.pos4), ptr(self.pos_last), ptr(self.maxd2), ptr(self.posq))
+
+    def fuses_next_first(self):
+        """Whether graph replays may let the force pass do the next step's first
+        half (the EAM global-list pass with one element type)."""
+        ff = self.force_field
+        return ff is not None and bool(getattr(ff, "fuses_next_first", lambda e: False)(self))
+
     def compute_forces(self, virial=True):
         if self.force_field is None:
             raise RuntimeError("no force field attached")
         self.force_field.compute(self, virial)
 
-    def first_half(self, records=False):
-        """B+A; records=True also writes the position records (posq)."""
+    def first_half(self, records=False, unfused_only=False):
+        """B+A; records=True also writes the position records (posq).
+        unfused_only: only the replicas whose first half the previous step's
+        force pass did not do (those with a barostat or loading)."""
         if records:
             call("sim_integrate_first_q", ptr(self.reps), ptr(self.tiles), self.ntiles,
                  self.types_struct, ptr(self.pos4), ptr(self.vel), ptr(self.force),
                  ptr(self.vel_half), ptr(self.pos_last), ptr(self.maxd2), ptr(self.posq),
-                 stream_ptr())
+                 int(bool(unfused_only)), stream_ptr())
             return
+        if unfused_only:
+            raise ValueError("unfused_only needs records=True")
         call("sim_integrate_first", ptr(self.reps), ptr(self.tiles), self.ntiles, self.types_struct,
              ptr(self.pos4), ptr(self.vel), ptr(self.force), ptr(self.vel_half),
              ptr(self.pos_last), ptr(self.maxd2), stream_ptr())
@@ -717,37 +739,85 @@ class ReplicaEngine:
 
     def step(self, virial=None, noise=None):
         """One MD step for every replica (md_model.py:103-153)."""
+        self._step(virial, noise)
+
+    def _step(self, virial=None, noise=None, first_unfused=False, fuse_next=False):
+        """step() as recorded in the graph replays of run(): first_unfused: the
+        previous step's force pass did the first half of the replicas without
+        barostat or loading; fuse_next: this step's force pass does the next
+        step's first half for them."""
         if virial is None:
             virial = True
         if self.loading:
             self.deform()
-        self.first_half(records=True)
-        self.build_and_forces(virial or self.has_barostat, noise, second=True)
+        self.first_half(records=True, unfused_only=first_unfused)
+        self._fuse_next = fuse_next
+        try:
+            self.build_and_forces(virial or self.has_barostat, noise, second=True)
+        finally:
+            self._fuse_next = False
         self.reduce(0, 15, advance=True)
         if self.has_barostat:
             self.barostat()
         self.steps_done += 1
 
-    def capture(self, virial=True):
-        """Record one step into a HIP graph. Recording executes nothing, so the
-        state is unchanged until the first replay."""
+    # graph replays: _graph (an unfused step) and, when the force field
+    # allows, three steps chained through the fused first half
+    # (first step, middle steps, last step), cleared together
+    @property
+    def _graph(self):
+        return self.__dict__.get("_graph_main")
+
+    @_graph.setter
+    def _graph(self, g):
+        self.__dict__["_graph_main"] = g
+        if g is None:
+            self.__dict__["_graphs_chain"] = {}
+
+    def _record(self, virial, first_unfused=False, fuse_next=False):
         if getattr(self, "_stale_slots", False) and self.use_tiled:
             self.build_list(force=True)     # refresh the LDS families' slots first
         g = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
-            self.step(virial=virial)
+            self._step(virial, first_unfused=first_unfused, fuse_next=fuse_next)
         self.steps_done -= 1
-        self._graph, self._graph_virial = g, virial
         torch.cuda.synchronize()
         return g
 
+    def capture(self, virial=True):
+        """Record one step into a HIP graph. Recording executes nothing, so the
+        state is unchanged until the first replay."""
+        g = self._record(virial)
+        self._graph, self._graph_virial = g, virial
+        return g
+
+    def _chain(self, virial):
+        """The three graphs of a run of >= 2 steps whose force passes do the
+        next step's first half: (first, middle, last)."""
+        chains = self.__dict__.setdefault("_graphs_chain", {})
+        if virial not in chains:
+            chains[virial] = (self._record(virial, False, True), self._record(virial, True, True),
+                              self._record(virial, True, False))
+        return chains[virial]
+
     def run(self, nsteps, graph=True, virial=True, check=True):
+        """nsteps steps. graph=True replays captured steps; with the EAM
+        global-list pass, a run of two or more steps lets each force pass but
+        the last do the next step's first half (the same bits as separate
+        steps; the state between runs is that of whole steps)."""
         if graph:
             if self._graph is None or self._graph_virial != virial:
                 self.capture(virial)
-            for _ in range(nsteps):
-                self._graph.replay()
+            if nsteps >= 2 and self.fuses_next_first():
+                g_first, g_mid, g_last = self._chain(virial)
+                g_first.replay()
+                for _ in range(nsteps - 2):
+                    g_mid.replay()
+                g_last.replay()
+            else:
+                for _ in range(nsteps):
+                    self._graph.replay()
             self.steps_done += nsteps
             self.profile["graph_replays"] += nsteps
         else:

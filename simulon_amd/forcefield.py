@@ -112,6 +112,12 @@ class EAMForceField:
 
     fuses_second = True
 
+    def fuses_next_first(self, eng):
+        """The fused force pass can also do the next step's first half: the
+        global-list pass (not a tiled family) with one element type."""
+        return (bool(self.fuses_second) and not eng.use_tiled and self.params.ntypes == 1
+                and len(eng.masses_amu) == 1 and eng.types_struct.ntypes == 1)
+
     def compute(self, eng, virial=True, force=None, partial=None, keep_rho=False, second=None):
         """force/partial: optional output arrays in place of the engine's
         (float [N][3] and double [ntiles][16]; the global-list kernels run).
@@ -149,7 +155,7 @@ class EAMForceField:
                 int(bool(virial)))
         if second is not None and own:
             call("sim_eam_force_second", *args, *second,
-                 ctypes.byref(st) if st is not None else None, s)
+                 ctypes.byref(st) if st is not None else None, *eng.next_first_args(), s)
             return True
         if st is not None:
             call("sim_eam_force_staged", *args, ctypes.byref(st), s)

@@ -4,7 +4,10 @@
 every thermo column agree bit for bit over graph-replayed and eager steps, for
 Langevin, NVE and Berendsen NPT replicas packed together, on the per-tile
 global tables (small launch) and on the persistent LDS-table kernel (more
-than 256 tiles).
+than 256 tiles). Graph runs of two or more steps also let the force pass do
+the next step's first half (the NPT replica's stays in its own kernel): the
+same bits as runs that do not, across list rebuilds and runs of 1, 2 and more
+steps.
 """
 import numpy as np
 import pytest
@@ -34,12 +37,14 @@ def _specs(sides):
     return out
 
 
-def _engine(setfl, sides, fused):
-    eng = ReplicaEngine(_specs(sides), [lat.W_MASS], cutoff=5.5, skin=1.0, seed=99,
+def _engine(setfl, sides, fused, skin=1.0, chain=True):
+    eng = ReplicaEngine(_specs(sides), [lat.W_MASS], cutoff=5.5, skin=skin, seed=99,
                         family="global")
     ff = EAMForceField(setfl, device=eng.device)
     if not fused:
         ff.fuses_second = False            # instance attribute: the separate kernel
+    if not chain:
+        ff.fuses_next_first = lambda e: False
     eng.force_field = ff
     eng.setup()
     return eng
@@ -61,3 +66,23 @@ def test_fused_second_half_bitwise(setfl, sides):
     ta, tb = a.thermo_host(), b.thermo_host()
     np.testing.assert_array_equal(ta, tb)
     assert np.all(np.isfinite(ta))
+
+
+@pytest.mark.parametrize("sides", [(6, 5, 7), (40, 12, 30)], ids=["per-tile", "persistent"])
+def test_next_first_half_chain_bitwise(setfl, sides):
+    a = _engine(setfl, sides, True, skin=0.3)
+    b = _engine(setfl, sides, True, skin=0.3, chain=False)
+    print(f"[case] chain, sides {sides}: {a.N} atoms, {a.ntiles} tiles", flush=True)
+    assert a.fuses_next_first() and not b.fuses_next_first()
+    nb0 = a.nbuild.clone()
+    for n in (2, 1, 25, 3):
+        a.run(n, graph=True)
+        b.run(n, graph=True)
+    a.run(2, graph=False)
+    b.run(2, graph=False)
+    torch.cuda.synchronize()
+    assert int((a.nbuild - nb0).min()) > 0, "no list rebuild in the compared steps"
+    for name in ("pos4", "vel", "vel_half", "force", "posq", "pos_last", "maxd2", "nbuild"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert torch.equal(x, y), f"{name} differs"
+    np.testing.assert_array_equal(a.thermo_host(), b.thermo_host())
