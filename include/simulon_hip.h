@@ -250,6 +250,21 @@ int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int n,
                  int* cell_atoms, int* nbr, int* nnbr, int max_nbr, int* overflow,
                  int precise, const sim_nl_reorder_t* ro, void* stream);
 
+/* sim_nl_trigger followed by sim_nl_build with a reorder, in four fewer
+ * launches: the skin test (flag, nbuild, the maxima cleared) runs inside the
+ * build's grid launch, the cell prefix of the staged list build is written
+ * with the cell scan into act (int[R+1] work array, the caller's), and the
+ * rank, second scatter and gather of the reorder are one launch. flag,
+ * nbuild, the lists, the order and every array end as after the two calls.
+ * tiles, cell_rep and ro are required. Same interfaces as those two calls
+ * (io_utils/reader.py:136-149 for the trigger). */
+int sim_nl_update(int R, void* reps, const int* tiles, int ntiles, int n, double* pos4,
+                  double* pos_last, double* maxd2, int* flag, int* nbuild, int force_all,
+                  const int* cell_rep, int ncells, int* atom_cell, int* atom_slot,
+                  int* cell_count, int* cell_start, int* cell_atoms, int* nbr, int* nnbr,
+                  int max_nbr, int* overflow, int precise, int* act,
+                  const sim_nl_reorder_t* ro, void* stream);
+
 /* Same build for a single system given as plain [N,3] coordinates, float32
  * (dtype 0) or float64 (dtype 1): the pair test uses exactly the reference's
  * arithmetic in that dtype (core/box.py:122-130, gpu_kdtree.py:130-138).

@@ -90,13 +90,7 @@ __global__ void k_trigger(int R, sim_replica_t* reps, unsigned long long* maxd2,
                           int* nbuild, int force_all) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    const sim_replica_t& P = reps[r];
-    const double d2 = __longlong_as_double((long long)maxd2[r]);
-    // pairs closer than r_l at the build stay inside the list while
-    // 2 max|dx| <= skin - max(0, 1 - mu_acc) r_l  (isotropic box scaling)
-    const double allow = 0.5 * (P.skin - fmax(0.0, 1.0 - P.mu_acc) * P.rl);
-    const bool trip = allow <= 0.0 || d2 > (P.mu_acc == 1.0 ? P.trig2 : allow * allow);
-    const int f = (force_all || trip) ? 1 : 0;
+    const int f = skin_trip(reps[r], maxd2[r], force_all) ? 1 : 0;
     flag[r] = f;
     if (nbuild) nbuild[r] += f;
     maxd2[r] = 0ull;

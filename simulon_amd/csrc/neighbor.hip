@@ -31,18 +31,49 @@ __device__ __forceinline__ bool rep_active(const int* flag, int r) {
     return flag == nullptr || flag[r] != 0;
 }
 
-__device__ void grid_of(sim_replica_t& P);
+__device__ void grid_cells(sim_replica_t& P);
+
+// The box of the build: the trigger measures box changes from it.
+__device__ __forceinline__ void reset_build_box(sim_replica_t& P) {
+    P.mu_acc = 1.0;
+    for (int k = 0; k < 9; ++k) P.hb_inv[k] = P.hinv[k];
+}
 
 // Cell grid of every rebuilding replica (threads r < R) and the cleared cell
 // counts of their reserved cell ranges (threads c < ncells), in one launch.
 __global__ void k_grid_clear(int R, sim_replica_t* reps, const int* flag, const int* cell_rep,
                              int ncells, int* cell_count) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < R && rep_active(flag, g)) grid_of(reps[g]);
+    if (g < R && rep_active(flag, g)) {
+        grid_cells(reps[g]);
+        reset_build_box(reps[g]);
+    }
     if (g < ncells && rep_active(flag, cell_rep ? cell_rep[g] : 0)) cell_count[g] = 0;
 }
 
-__device__ void grid_of(sim_replica_t& P) {
+// k_grid_clear with the skin trigger (k_trigger) folded in: thread r < R
+// decides replica r's flag, counts the build and computes its grid; a cell's
+// thread takes the same decision from the same inputs (the displacement
+// maximum and the replica's skin, r_l and mu_acc, none of which this launch
+// writes: the grid touches only ncell, and mu_acc / hb_inv are reset and the
+// maximum cleared by the scan that follows).
+__global__ void k_grid_trigger(int R, sim_replica_t* reps, const unsigned long long* maxd2,
+                               int* flag, int* nbuild, int force_all, const int* cell_rep,
+                               int ncells, int* cell_count) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < R) {
+        const int f = skin_trip(reps[g], maxd2[g], force_all) ? 1 : 0;
+        flag[g] = f;
+        if (nbuild) nbuild[g] += f;
+        if (f) grid_cells(reps[g]);
+    }
+    if (g < ncells) {
+        const int r = cell_rep[g];
+        if (skin_trip(reps[r], maxd2[r], force_all)) cell_count[g] = 0;
+    }
+}
+
+__device__ void grid_cells(sim_replica_t& P) {
     const double* h = P.h;
     // perpendicular widths: V / |b x c|, V / |c x a|, V / |a x b|
     double a[3] = {h[0], h[1], h[2]}, b[3] = {h[3], h[4], h[5]}, c[3] = {h[6], h[7], h[8]};
@@ -68,8 +99,6 @@ __device__ void grid_of(sim_replica_t& P) {
     P.ncell[0] = nc[0];
     P.ncell[1] = nc[1];
     P.ncell[2] = nc[2];
-    P.mu_acc = 1.0;
-    for (int k = 0; k < 9; ++k) P.hb_inv[k] = P.hinv[k];
 }
 
 
@@ -139,13 +168,25 @@ __global__ void __launch_bounds__(1024) k_scan_flat(const int* cnt, int n, int* 
 // over its own cell range: replica r's atoms occupy [atom_base, atom_base +
 // natoms), so its cell starts begin at atom_base and the other replicas'
 // starts stay valid.
-__global__ void __launch_bounds__(1024) k_scan(const sim_replica_t* reps, const int* cnt,
-                                               int* start, const int* flag) {
+__device__ __forceinline__ bool stencil3(const sim_replica_t& P);
+
+// With the trigger folded into the grid launch (trig != nullptr: the
+// displacement maxima), workgroup r also clears replica r's maximum and, when
+// rebuilding it, resets its build box; workgroup 0 also writes act (the cell
+// prefix of the staged list build, act_base's work) when act != nullptr.
+__device__ void act_prefix(int R, const sim_replica_t* reps, const int* flag, int* act);
+
+__global__ void __launch_bounds__(1024) k_scan(sim_replica_t* reps, const int* cnt, int* start,
+                                               const int* flag, unsigned long long* trig,
+                                               int R, int* act) {
     __shared__ int wsum[16];
     __shared__ int carry;
     const int r = blockIdx.x;
+    if (trig != nullptr && threadIdx.x == 0) trig[r] = 0ull;
+    if (act != nullptr && r == 0) act_prefix(R, reps, flag, act);
     if (!rep_active(flag, r)) return;
-    const sim_replica_t& P = reps[r];
+    sim_replica_t& P = reps[r];
+    if (trig != nullptr && threadIdx.x == 0) reset_build_box(P);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int cb = P.cell_base, n = P.cell_cap;
     if (tid == 0) carry = P.atom_base;
@@ -282,6 +323,55 @@ __global__ void k_permute_gather(const sim_replica_t* reps, const SimTile* tiles
     }
     order_t[s] = order[a];
     cell_t[s] = atom_cell[a];
+}
+
+// k_cell_rank, the second k_scatter and k_permute_gather in one launch: atom
+// i (old slot) takes slot s = cell_start[c] + (its rank by index among the
+// atoms of its cell c, from the first scatter) and writes its wrapped
+// position, image, velocities, force, caller index and cell to the work
+// arrays at s. The gather form reads from cell_atoms[s] = i after the second
+// scatter, so both write the same values to the same slots; cell_atoms ends
+// as the identity either way (k_permute_store).
+__global__ void k_rank_permute(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
+                               const int* atom_cell, const int* cell_start, const int* cell_count,
+                               const int* cell_atoms, const double4* pos, const float* vel,
+                               const float* force, const long long* order, const int4* image,
+                               double4* pos_t, float* vel_t, float* force_t, long long* order_t,
+                               int4* image_t, int* cell_t, const float* vh, float* vh_t) {
+    const SimTile t = tiles[blockIdx.x];
+    if (!rep_active(flag, t.rep)) return;
+    const sim_replica_t& P = reps[t.rep];
+    const int i = t.start + threadIdx.x;
+    if (i >= P.atom_base + P.natoms) return;
+    const int c = atom_cell[i];
+    const int c0 = cell_start[c];
+    const int* a = cell_atoms + c0;
+    const int n = cell_count[c];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += a[k] < i ? 1 : 0;
+    const int s = c0 + rank;
+    double4 x = pos[i];
+    const double* hi = P.hinv;
+    const double* h = P.h;
+    const double f0 = floor(x.x * hi[0] + x.y * hi[3] + x.z * hi[6]);
+    const double f1 = floor(x.x * hi[1] + x.y * hi[4] + x.z * hi[7]);
+    const double f2 = floor(x.x * hi[2] + x.y * hi[5] + x.z * hi[8]);
+    x.x -= f0 * h[0] + f1 * h[3] + f2 * h[6];
+    x.y -= f0 * h[1] + f1 * h[4] + f2 * h[7];
+    x.z -= f0 * h[2] + f1 * h[5] + f2 * h[8];
+    pos_t[s] = x;
+    int4 im = image[i];
+    im.x += (int)f0;
+    im.y += (int)f1;
+    im.z += (int)f2;
+    image_t[s] = im;
+    for (int k = 0; k < 3; ++k) {
+        vel_t[(size_t)s * 3 + k] = vel[(size_t)i * 3 + k];
+        force_t[(size_t)s * 3 + k] = force[(size_t)i * 3 + k];
+        vh_t[(size_t)s * 3 + k] = vh[(size_t)i * 3 + k];
+    }
+    order_t[s] = order[i];
+    cell_t[s] = c;
 }
 
 __global__ void k_permute_store(const sim_replica_t* reps, const SimTile* tiles, const int* flag,
@@ -477,13 +567,14 @@ __global__ void __launch_bounds__(SIM_TILE)
 #endif
 
 // act[r] = cells before replica r of the rebuilding replicas built here;
-// act[R] = their total. One workgroup of 256 threads.
-__global__ void __launch_bounds__(256)
-    k_act_base(int R, const sim_replica_t* reps, const int* flag, int* act) {
-    __shared__ int wsum[4];
+// act[R] = their total. Called by every thread of one workgroup (a multiple
+// of 64 threads, at most 1,024).
+__device__ void act_prefix(int R, const sim_replica_t* reps, const int* flag, int* act) {
+    __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int nt = blockDim.x, nw = nt >> 6;
     int carry = 0;
-    for (int base = 0; base < R; base += 256) {
+    for (int base = 0; base < R; base += nt) {
         const int r = base + tid;
         int v = 0;
         if (r < R && rep_active(flag, r) && stencil3(reps[r]))
@@ -496,14 +587,21 @@ __global__ void __launch_bounds__(256)
         }
         if (lane == 63) wsum[wid] = incl;
         __syncthreads();
-        int before = 0;
-        for (int w = 0; w < wid; ++w) before += wsum[w];
-        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        int before = 0, total = 0;
+        for (int w = 0; w < nw; ++w) {
+            before += w < wid ? wsum[w] : 0;
+            total += wsum[w];
+        }
         if (r < R) act[r] = carry + before + incl - v;
         carry += total;
         __syncthreads();
     }
     if (tid == 0) act[R] = carry;
+}
+
+__global__ void __launch_bounds__(256)
+    k_act_base(int R, const sim_replica_t* reps, const int* flag, int* act) {
+    act_prefix(R, reps, flag, act);
 }
 
 // Waves per SIMD the staged build's registers are held to (5: 96 VGPRs, no
@@ -726,37 +824,71 @@ __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
     }
 }
 
+// The skin trigger folded into sim_nl_update's build (k_grid_trigger).
+struct NlTrigger {
+    unsigned long long* maxd2;
+    int* flag;
+    int* nbuild;
+    int force_all;
+    int* act;   // [R+1] cell prefix of the staged list build
+};
+
 template <typename TC, typename T, int STRIDE>
 int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, const T* x,
                double* pos_last, const int* flag, const int* cell_rep, int ncells, int* atom_cell,
                int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms, int* nbr,
                int* nnbr, int n_stride, int max_nbr, int* overflow, hipStream_t st,
-               const sim_nl_reorder_t* ro = nullptr) {
-    hipLaunchKernelGGL(k_grid_clear, dim3((max(ncells, R) + 255) / 256), dim3(256), 0, st, R,
-                       reps, flag, cell_rep, ncells, cell_count);
+               const sim_nl_reorder_t* ro = nullptr, const NlTrigger* trig = nullptr) {
+    const bool perm = ro != nullptr && tiles != nullptr;
+    if (perm && (!ro->vel || !ro->force || !ro->order || !ro->slot_of || !ro->image ||
+                 !ro->pos_tmp || !ro->vel_tmp || !ro->force_tmp || !ro->order_tmp ||
+                 !ro->image_tmp || !ro->cell_tmp || !ro->vel_half || !ro->vel_half_tmp))
+        return SIM_E_ARG;
+    // the fused form (trigger in the grid launch, act with the scan, rank and
+    // gather in one launch) needs the reorder
+    if (trig != nullptr && (!perm || cell_rep == nullptr)) return SIM_E_ARG;
+    const dim3 gc((max(ncells, R) + 255) / 256);
+    if (trig != nullptr)
+        hipLaunchKernelGGL(k_grid_trigger, gc, dim3(256), 0, st, R, reps, trig->maxd2, trig->flag,
+                           trig->nbuild, trig->force_all, cell_rep, ncells, cell_count);
+    else
+        hipLaunchKernelGGL(k_grid_clear, gc, dim3(256), 0, st, R, reps, flag, cell_rep, ncells,
+                           cell_count);
     hipLaunchKernelGGL((k_bin<T, STRIDE>), dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, x,
                        flag, cell_count, atom_cell, atom_slot);
-    hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag);
+    if (trig != nullptr)
+        hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag,
+                           trig->maxd2, R, trig->act);
+    else
+        hipLaunchKernelGGL(k_scan, dim3(R), dim3(1024), 0, st, reps, cell_count, cell_start, flag,
+                           nullptr, 0, nullptr);
     if (ro != nullptr && ro->row_base != nullptr)
         hipLaunchKernelGGL(k_row_base, dim3(1), dim3(256), 0, st, R, reps, flag, ro->row_base);
     hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                        atom_cell, atom_slot, cell_start, cell_atoms);
-    hipLaunchKernelGGL(k_cell_rank, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
-                       atom_cell, cell_start, cell_count, cell_atoms, atom_slot);
-    hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
-                       atom_cell, atom_slot, cell_start, cell_atoms);
-    if (ro != nullptr && tiles != nullptr) {
-        if (!ro->vel || !ro->force || !ro->order || !ro->slot_of || !ro->image || !ro->pos_tmp ||
-            !ro->vel_tmp || !ro->force_tmp || !ro->order_tmp || !ro->image_tmp || !ro->cell_tmp ||
-            !ro->vel_half || !ro->vel_half_tmp)
-            return SIM_E_ARG;
-        auto* P4 = reinterpret_cast<double4*>(const_cast<T*>(x));
-        hipLaunchKernelGGL(k_permute_gather, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
-                           cell_atoms, atom_cell, P4, ro->vel, ro->force, ro->order,
-                           reinterpret_cast<const int4*>(ro->image),
+    auto* P4 = reinterpret_cast<double4*>(const_cast<T*>(x));
+    if (trig != nullptr) {
+        hipLaunchKernelGGL(k_rank_permute, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                           atom_cell, cell_start, cell_count, cell_atoms, P4, ro->vel, ro->force,
+                           ro->order, reinterpret_cast<const int4*>(ro->image),
                            reinterpret_cast<double4*>(ro->pos_tmp), ro->vel_tmp, ro->force_tmp,
                            ro->order_tmp, reinterpret_cast<int4*>(ro->image_tmp), ro->cell_tmp,
                            ro->vel_half, ro->vel_half_tmp);
+    } else {
+        hipLaunchKernelGGL(k_cell_rank, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                           atom_cell, cell_start, cell_count, cell_atoms, atom_slot);
+        hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
+                           atom_cell, atom_slot, cell_start, cell_atoms);
+        if (perm)
+            hipLaunchKernelGGL(k_permute_gather, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles,
+                               flag, cell_atoms, atom_cell, P4, ro->vel, ro->force, ro->order,
+                               reinterpret_cast<const int4*>(ro->image),
+                               reinterpret_cast<double4*>(ro->pos_tmp), ro->vel_tmp,
+                               ro->force_tmp, ro->order_tmp,
+                               reinterpret_cast<int4*>(ro->image_tmp), ro->cell_tmp,
+                               ro->vel_half, ro->vel_half_tmp);
+    }
+    if (perm) {
         hipLaunchKernelGGL(k_permute_store, dim3(ntiles), dim3(SIM_TILE), 0, st, reps, tiles, flag,
                            reinterpret_cast<const double4*>(ro->pos_tmp), ro->vel_tmp,
                            ro->force_tmp, ro->order_tmp,
@@ -767,12 +899,15 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
     }
     if (ro != nullptr && tiles != nullptr) {   // positions wrapped and in cell order
         // staged build for replicas with >= 3 cells per dimension (their cell
-        // prefix in atom_slot, free after the scatter); k_list for the rest
-        const bool staged = n_stride >= R + 1;
+        // prefix in act, or in atom_slot, free after the scatter); k_list for
+        // the rest
+        const bool staged = trig != nullptr || n_stride >= R + 1;
         if (staged) {
-            hipLaunchKernelGGL(k_act_base, dim3(1), dim3(256), 0, st, R, reps, flag, atom_slot);
+            int* act = trig != nullptr ? trig->act : atom_slot;
+            if (trig == nullptr)
+                hipLaunchKernelGGL(k_act_base, dim3(1), dim3(256), 0, st, R, reps, flag, act);
             hipLaunchKernelGGL((k_list_staged<TC, T, STRIDE>), dim3(SIM_LIST_GRID), dim3(SIM_TILE),
-                               0, st, R, reps, atom_slot, x, cell_count, cell_start, nbr, nnbr,
+                               0, st, R, reps, act, x, cell_count, cell_start, nbr, nnbr,
                                n_stride, max_nbr, overflow, pos_last, ro->nbr16, ro->stencil_max);
         }
         if (!(staged && ro->stencil3_only))
@@ -853,6 +988,31 @@ extern "C" int sim_nl_build(int R, void* reps, const int* tiles, int ntiles, int
     return build_impl<float, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
                                         atom_cell, atom_slot, cell_count, cell_start, cell_atoms,
                                         nbr, nnbr, n, max_nbr, overflow, as_stream(stream), ro);
+}
+
+extern "C" int sim_nl_update(int R, void* reps, const int* tiles, int ntiles, int n, double* pos4,
+                             double* pos_last, double* maxd2, int* flag, int* nbuild,
+                             int force_all, const int* cell_rep, int ncells, int* atom_cell,
+                             int* atom_slot, int* cell_count, int* cell_start, int* cell_atoms,
+                             int* nbr, int* nnbr, int max_nbr, int* overflow, int precise,
+                             int* act, const sim_nl_reorder_t* ro, void* stream) {
+    if (R <= 0 || !reps || !tiles || ntiles <= 0 || n <= 0 || !pos4 || !maxd2 || !flag ||
+        !cell_rep || ncells <= 0 || !atom_cell || !atom_slot || !cell_count || !cell_start ||
+        !cell_atoms || !nbr || !nnbr || max_nbr <= 0 || !overflow || !act || !ro)
+        return SIM_E_ARG;
+    auto* P = static_cast<sim_replica_t*>(reps);
+    auto* T = reinterpret_cast<const SimTile*>(tiles);
+    const NlTrigger trig{reinterpret_cast<unsigned long long*>(maxd2), flag, nbuild,
+                         force_all ? 1 : 0, act};
+    if (precise)
+        return build_impl<double, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep,
+                                             ncells, atom_cell, atom_slot, cell_count, cell_start,
+                                             cell_atoms, nbr, nnbr, n, max_nbr, overflow,
+                                             as_stream(stream), ro, &trig);
+    return build_impl<float, double, 4>(R, P, T, ntiles, pos4, pos_last, flag, cell_rep, ncells,
+                                        atom_cell, atom_slot, cell_count, cell_start, cell_atoms,
+                                        nbr, nnbr, n, max_nbr, overflow, as_stream(stream), ro,
+                                        &trig);
 }
 
 extern "C" int sim_nl_build_xyz(void* rep, const void* xyz, int dtype, int n, int* atom_cell,

@@ -279,6 +279,19 @@ __device__ __forceinline__ double first_half_atom(const sim_replica_t& P, float 
     return (ex * ex + ey * ey) + ez * ez;
 }
 
+// Skin trigger of one replica (io_utils/reader.py:136-149): rebuild when the
+// largest displacement since the last build, m (bits of a non-negative
+// double), exceeds what the list's skin still covers. Pairs closer than r_l
+// at the build stay inside the list while
+//   2 max|dx| <= skin - max(0, 1 - mu_acc) r_l   (isotropic box scaling).
+__device__ __forceinline__ bool skin_trip(const sim_replica_t& P, unsigned long long m,
+                                          int force_all) {
+    const double d2 = __longlong_as_double((long long)m);
+    const double allow = 0.5 * (P.skin - fmax(0.0, 1.0 - P.mu_acc) * P.rl);
+    const bool trip = allow <= 0.0 || d2 > (P.mu_acc == 1.0 ? P.trig2 : allow * allow);
+    return force_all || trip;
+}
+
 // Replicas whose next first half the fused EAM force pass may do: no
 // barostat and no loading step between this step's end and the next step's
 // first half.

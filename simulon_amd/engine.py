@@ -454,6 +454,7 @@ class ReplicaEngine:
         self.overflow = torch.zeros(1, **i32)
         self.flag = torch.ones(R, **i32)
         self.nbuild = torch.zeros(R, **i32)
+        self.act = torch.zeros(R + 1, **i32)      # cell prefix of the staged list build
         self.maxd2 = torch.zeros(R, dtype=torch.float64, device=dev)
         self.partial = torch.zeros((self.ntiles, _lib.SIM_NPART), dtype=torch.float64, device=dev)
         self.thermo = torch.zeros((R, _lib.SIM_NPART), dtype=torch.float64, device=dev)
@@ -529,13 +530,14 @@ class ReplicaEngine:
         # _grid_of while no box changes)
         self._ro.stencil3_only = int(self.can_tile and self.box_fixed and not self.has_barostat
                                      and not self.loading)
-        call("sim_nl_trigger", self.R, ptr(self.reps), ptr(self.maxd2), ptr(self.flag),
-             ptr(self.nbuild), int(bool(force)), s)
-        call("sim_nl_build", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
-             ptr(self.pos4), ptr(self.pos_last), ptr(self.flag), ptr(self.cell_rep), self.ncells,
+        # the skin trigger and the gated rebuild in one call (sim_nl_trigger +
+        # sim_nl_build in fewer launches)
+        call("sim_nl_update", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles, self.N,
+             ptr(self.pos4), ptr(self.pos_last), ptr(self.maxd2), ptr(self.flag),
+             ptr(self.nbuild), int(bool(force)), ptr(self.cell_rep), self.ncells,
              ptr(self.atom_cell), ptr(self.atom_slot), ptr(self.cell_count), ptr(self.cell_start),
              ptr(self.cell_atoms), ptr(self.nbr), ptr(self.nnbr), self.max_nbr,
-             ptr(self.overflow), int(self.precise_list), ctypes.byref(self._ro), s)
+             ptr(self.overflow), int(self.precise_list), ptr(self.act), ctypes.byref(self._ro), s)
         if self._stage is not None:
             self._stage_build(self.flag)
 
@@ -744,13 +746,15 @@ class ReplicaEngine:
     def _step(self, virial=None, noise=None, first_unfused=False, fuse_next=False):
         """step() as recorded in the graph replays of run(): first_unfused: the
         previous step's force pass did the first half of the replicas without
-        barostat or loading; fuse_next: this step's force pass does the next
-        step's first half for them."""
+        barostat or loading (no first-half launch at all when every replica is
+        such); fuse_next: this step's force pass does the next step's first
+        half for them."""
         if virial is None:
             virial = True
         if self.loading:
             self.deform()
-        self.first_half(records=True, unfused_only=first_unfused)
+        if not (first_unfused and self._all_fusable()):
+            self.first_half(records=True, unfused_only=first_unfused)
         self._fuse_next = fuse_next
         try:
             self.build_and_forces(virial or self.has_barostat, noise, second=True)
@@ -773,6 +777,7 @@ class ReplicaEngine:
         self.__dict__["_graph_main"] = g
         if g is None:
             self.__dict__["_graphs_chain"] = {}
+            self.__dict__["_fusable_all"] = None
 
     def _record(self, virial, first_unfused=False, fuse_next=False):
         if getattr(self, "_stale_slots", False) and self.use_tiled:
@@ -792,11 +797,25 @@ class ReplicaEngine:
         self._graph, self._graph_virial = g, virial
         return g
 
+    def _all_fusable(self):
+        """Whether no replica record asks for a barostat or loading step (the
+        records as the chain graphs were recorded; write_replica_state clears
+        those graphs when a record asks for either)."""
+        v = self.__dict__.get("_fusable_all")
+        if v is None:
+            raw = self.reps.cpu().numpy().tobytes()
+            sz = ctypes.sizeof(_lib.Replica)
+            recs = [_lib.Replica.from_buffer_copy(raw[r * sz:(r + 1) * sz]) for r in range(self.R)]
+            v = all(P.barostat == 0 and P.loading == 0 for P in recs)
+            self.__dict__["_fusable_all"] = v
+        return v
+
     def _chain(self, virial):
         """The three graphs of a run of >= 2 steps whose force passes do the
         next step's first half: (first, middle, last)."""
         chains = self.__dict__.setdefault("_graphs_chain", {})
         if virial not in chains:
+            self._all_fusable()        # host read of the records, before recording
             chains[virial] = (self._record(virial, False, True), self._record(virial, True, True),
                               self._record(virial, True, False))
         return chains[virial]
@@ -903,6 +922,10 @@ class ReplicaEngine:
         raw = torch.frombuffer(bytearray(bytes(P)), dtype=torch.uint8)
         self.reps[r * sz:(r + 1) * sz].copy_(raw.to(self.device))
         self.specs[r].dt = P.dt
+        if (P.barostat or P.loading) and self.__dict__.get("_fusable_all"):
+            # the chain graphs were recorded without a first-half launch
+            self.__dict__["_graphs_chain"] = {}
+            self.__dict__["_fusable_all"] = None
 
     def box(self, r=0):
         return np.array(self.replica_state(r).h[:], np.float64).reshape(3, 3)
