@@ -51,19 +51,22 @@ __device__ __forceinline__ void displacement(const double4& xi, const double4& x
 //
 // The wrapped 32-bit difference of two records is s_i - s_j - round(s_i - s_j)
 // (the fractional minimum image of core/box.py:109-130) exactly, away from a
-// tie at half a box vector; it is scaled by 2^-32 and mapped through H in
-// float64, then rounded once to float32. Quantisation moves an atom by at
-// most 2^-33 of each box vector (4e-8 A in a 340 A box), far below the
-// float32 rounding of the displacement that follows.
-// H scaled by 2^-32 (exactly): maps a 32-bit record difference to Cartesian.
+// tie at half a box vector. It is converted to float32 (one rounding, 2^-24
+// relative) and mapped through H * 2^-32 held in float32 (one rounding of
+// each entry, a common scale error below 6e-8 of every distance): the
+// displacement comes within about one float32 ulp of the float64 value, at
+// two VALU instructions per component of an orthogonal box instead of three
+// in float64. Quantisation moves an atom by at most 2^-33 of each box vector
+// (4e-8 A in a 340 A box), far below that rounding.
+// H scaled by 2^-32, rounded to float32: maps a record difference to Cartesian.
 struct QBox {
-    double h[9];
+    float h[9];
 };
 
 __device__ __forceinline__ QBox load_qbox(const sim_replica_t& P) {
     QBox b;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) b.h[k] = P.h[k] * SIM_Q_INV;
+    for (int k = 0; k < 9; ++k) b.h[k] = (float)(P.h[k] * SIM_Q_INV);
     return b;
 }
 
@@ -74,6 +77,44 @@ __device__ __forceinline__ QBox load_qbox(const sim_replica_t& P) {
 template <int GEO>
 __device__ __forceinline__ void qdisp(const int4& a, const int4& b, const QBox& B, float& rx,
                                       float& ry, float& rz) {
+    const float s0 = (float)(int)((unsigned)a.x - (unsigned)b.x);
+    const float s1 = (float)(int)((unsigned)a.y - (unsigned)b.y);
+    const float s2 = (float)(int)((unsigned)a.z - (unsigned)b.z);
+    if (GEO == 1) {
+        rx = s0 * B.h[0];
+        ry = s1 * B.h[4];
+        rz = s2 * B.h[8];
+    } else if (GEO == 2) {
+        rx = __builtin_fmaf(s2, B.h[6], __builtin_fmaf(s1, B.h[3], s0 * B.h[0]));
+        ry = __builtin_fmaf(s2, B.h[7], s1 * B.h[4]);
+        rz = s2 * B.h[8];
+    } else {
+        rx = __builtin_fmaf(s2, B.h[6], __builtin_fmaf(s1, B.h[3], s0 * B.h[0]));
+        ry = __builtin_fmaf(s2, B.h[7], __builtin_fmaf(s1, B.h[4], s0 * B.h[1]));
+        rz = __builtin_fmaf(s2, B.h[8], __builtin_fmaf(s1, B.h[5], s0 * B.h[2]));
+    }
+}
+
+// The same displacement formed in float64 and rounded once to float32 (three
+// VALU instructions per component of an orthogonal box). The LJ kernels use
+// it: their virial sums pair terms of both signs that cancel to a few parts in
+// a hundred, which magnifies the float32 form's common scale error (the
+// rounded H * 2^-32) about two hundred times; on a 1,000-atom argon cell the
+// float32 form puts the virial 7.6e-6 (relative) from float64, this one 5e-8.
+struct QBoxD {
+    double h[9];
+};
+
+__device__ __forceinline__ QBoxD load_qbox_d(const sim_replica_t& P) {
+    QBoxD b;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.h[k] = P.h[k] * SIM_Q_INV;
+    return b;
+}
+
+template <int GEO>
+__device__ __forceinline__ void qdisp_d(const int4& a, const int4& b, const QBoxD& B, float& rx,
+                                        float& ry, float& rz) {
     const double s0 = (double)(int)((unsigned)a.x - (unsigned)b.x);
     const double s1 = (double)(int)((unsigned)a.y - (unsigned)b.y);
     const double s2 = (double)(int)((unsigned)a.z - (unsigned)b.z);

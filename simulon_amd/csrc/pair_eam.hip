@@ -592,6 +592,10 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
         if (INTEG && active) {
             ty_i = S.ty.ntypes == 1 ? 0 : typ[i].w;
             w_o = second_half_pre(P, i, ty_i, S.vh, S.noise, S.key, S.seed, step);
+            if (next) {   // loaded before the walk, which hides their latency
+                xn = S.pos[i];
+                ln = S.pos_last[i];
+            }
         }
         if (active) {
             const QBox B = load_qbox(P);
@@ -610,10 +614,6 @@ __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
             } else {
                 force_atom<NT, LT, VIRIAL, false>(a, i, xi, ti, cnt, cmin, cmax, B, rec, typ, nbr,
                                                   n_stride, E, tl, thr);
-            }
-            if (next) {   // issued before the stores below, so their latency overlaps them
-                xn = S.pos[i];
-                ln = S.pos_last[i];
             }
             force[(size_t)i * 3 + 0] = a.fx;
             force[(size_t)i * 3 + 1] = a.fy;

@@ -100,7 +100,7 @@ struct LJRow {
 // the end of the list (self) fail the cutoff test.
 template <int UB, bool MASKED, bool VIRIAL, int MODE, bool NT1, int GEO>
 __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, int cnt,
-                                         const int4 xi, const LJRow<NT1>& row, const QBox& B,
+                                         const int4 xi, const LJRow<NT1>& row, const QBoxD& B,
                                          const int4* __restrict__ pos, const sim_lj_params_t& lj,
                                          float thr) {
     int4 xj[UB];
@@ -109,7 +109,7 @@ __device__ __forceinline__ void lj_batch(LJAcc& a, const int (&j)[UB], int k, in
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
         float rx, ry, rz;
-        qdisp<GEO>(xi, xj[u], B, rx, ry, rz);
+        qdisp_d<GEO>(xi, xj[u], B, rx, ry, rz);
         const float r2 = (!MASKED || k + u < cnt) ? sq3(rx, ry, rz) : thr;
         lj_pair<VIRIAL, MODE>(a, rx, ry, rz, r2, thr, row.e(xj[u].w), row.s(xj[u].w), lj);
     }
@@ -125,7 +125,7 @@ template <bool VIRIAL, int MODE, bool NT1, int GEO>
 // The wave walks its lists in lock step (k wave-uniform, up to cmax, the
 // wave's longest list; batches past cmin, its shortest, are masked).
 __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt, int cmin,
-                                        int cmax, const QBox& B, const int4* __restrict__ pos,
+                                        int cmax, const QBoxD& B, const int4* __restrict__ pos,
                                         const int* __restrict__ nbr, int n_stride,
                                         const sim_lj_params_t& lj, float thr) {
     constexpr int U = SIM_LJ_BATCH;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
     const int cnt = active ? nnbr[i] : 0;
     const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
     if (active) {
-        const QBox B = load_qbox(P);
+        const QBoxD B = load_qbox_d(P);
         const int4 xi = pos[i];
         const int geo = box_geo(P);
         if (geo == 1)
