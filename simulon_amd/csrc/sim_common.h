@@ -176,20 +176,24 @@ __device__ __forceinline__ float u01(uint32_t b) {
 
 // Three N(0,1) draws for (atom, step) by Box-Muller on one Philox block.
 // tag separates independent streams (atoms: the default; barostat: its own).
+// The transcendentals are the hardware's: v_log_f32 (log2), v_sqrt_f32 and
+// v_sin_f32 / v_cos_f32, which take the angle in turns, so sin(2 pi u) is one
+// instruction on u (their errors, about one float32 ulp for the logarithm
+// and square root and below 1e-6 absolute for sine and cosine, are far
+// below what a sample of N(0,1) noise can show); the library's logf, sqrtf
+// and sincospif took about 120 VALU instructions per atom.
 __device__ __forceinline__ void gauss3(unsigned long long seed, unsigned long long step,
                                        uint32_t atom, float& g0, float& g1, float& g2,
                                        uint32_t tag = 0x5eedu) {
     u32x4 c{atom, (uint32_t)step, (uint32_t)(step >> 32), tag};
     u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    float ra = sqrtf(-2.0f * logf(u01(r.x)));
-    float rb = sqrtf(-2.0f * logf(u01(r.z)));
-    float sa, ca, sb, cb;
-    sincospif(2.0f * u01(r.y), &sa, &ca);
-    sincospif(2.0f * u01(r.w), &sb, &cb);
-    g0 = ra * ca;
-    g1 = ra * sa;
-    g2 = rb * cb;
-    (void)sb;
+    const float m2ln2 = -1.3862943611198906f;   // -2 ln 2: -2 ln u = -2 ln2 log2 u
+    const float ra = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01(r.x)));
+    const float rb = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01(r.z)));
+    const float ta = u01(r.y), tb = u01(r.w);
+    g0 = ra * __builtin_amdgcn_cosf(ta);
+    g1 = ra * __builtin_amdgcn_sinf(ta);
+    g2 = rb * __builtin_amdgcn_cosf(tb);
 }
 
 // Second half of the velocity-Verlet step for atom i (core/integrator/

@@ -34,7 +34,7 @@
 // Every hash probe is bounded by the table size. Groups whose tiles all
 // belong to replicas with flag 0 keep their sets (their lists did not
 // change).
-#include "sim_common.h"
+#include "pair_common.h"
 
 namespace {
 
@@ -278,10 +278,9 @@ __global__ void __launch_bounds__(NT)
         if (l == 0) cntl[a] = cnt;
         if (t == 0) s_bad = 0;
         __syncthreads();
-        if (t < SIM_TILE / 64) {   // longest list of each wave (64 atoms) of the tile
-            int m = 0;
-            for (int q = 0; q < 64; ++q) m = max(m, cntl[t * 64 + q]);
-            wmax[t] = m;
+        if (t < SIM_TILE) {   // longest list of each wave (64 atoms) of the tile: waves 0-3
+            const int m = wave_max_int(cntl[t]);
+            if ((t & 63) == 0) wmax[t >> 6] = m;
         }
         __syncthreads();
         const int pad_end = min((wmax[a / 64] + 7) & ~7, max_nbr);
@@ -289,7 +288,8 @@ __global__ void __launch_bounds__(NT)
             const int hs = find<UC>(tkey, atom_cell[i]);
             if (hs < 0) s_bad = 1;
             const int self = hs < 0 ? 0 : toff[hs] + (i - tstart[hs]);
-            constexpr int B = 8;
+            // 24 entries per thread in flight: one pass for lists up to 96
+            constexpr int B = 24;
             for (int k0 = l; k0 < cnt; k0 += LPA * B) {
                 int j[B], c[B];
     #pragma unroll
