@@ -565,6 +565,9 @@ __global__ void __launch_bounds__(SIM_TILE)
 #ifndef SIM_LIST_STAGE_F32
 #define SIM_LIST_STAGE_F32 1
 #endif
+// Replicas whose cell prefix (act) the staged build holds in LDS for its
+// search of a cell's replica; more replicas search act in global memory.
+#define SIM_LIST_ACT_LDS 512
 
 // act[r] = cells before replica r of the rebuilding replicas built here;
 // act[R] = their total. Called by every thread of one workgroup (a multiple
@@ -609,7 +612,8 @@ __global__ void __launch_bounds__(256)
 #ifndef SIM_LIST_WAVES
 #define SIM_LIST_WAVES 5
 #endif
-template <typename TC, typename T, int STRIDE>
+// ACT_LDS: R <= SIM_LIST_ACT_LDS, the replica search reads act from LDS.
+template <typename TC, typename T, int STRIDE, bool ACT_LDS>
 __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
     k_list_staged(int R, const sim_replica_t* reps, const int* act, const T* x,
                   const int* cell_count, const int* cell_start, int* nbr, int* nnbr, int n_stride,
@@ -626,15 +630,23 @@ __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
     __shared__ int mstart[27], mcount[27], mpre[28], mslot[27], mcol[3];
     __shared__ TC msh[27][3];
     __shared__ double mref[3];
+    __shared__ int sact[ACT_LDS ? SIM_LIST_ACT_LDS + 1 : 1];
     const int tid = threadIdx.x;
     const int total = act[R];
+    if ((int)blockIdx.x >= total) return;   // uniform: no cell for this workgroup
+    // the replica search of every cell below reads act from LDS (one load
+    // round trip per workgroup instead of one per probe)
+    if (ACT_LDS) {
+        for (int r = tid; r <= R; r += SIM_TILE) sact[r] = act[r];
+        __syncthreads();
+    }
     const double margin = sizeof(TC) == 8 ? 1e-9 : 1e-5;   // as k_list
     const double smargin = sizeof(QT) == 8 ? 1e-9 : 1e-5;
     for (int ac = blockIdx.x; ac < total; ac += gridDim.x) {
         int lo = 0, hi = R - 1;   // last replica with act[r] <= ac
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (act[mid] <= ac) lo = mid;
+            if ((ACT_LDS ? sact[mid] : act[mid]) <= ac) lo = mid;
             else hi = mid - 1;
         }
         const sim_replica_t& P = reps[lo];
@@ -669,8 +681,11 @@ __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
             mcol[0] = coff[0];
             mcol[1] = coff[1];
             mcol[2] = coff[2];
-            for (int c = 0; c < 3; ++c)
-                mref[c] = mcount[13] > 0 ? (double)x[(size_t)mstart[13] * STRIDE + c] : 0.0;
+            // reference point of the float offsets: the home cell's lower
+            // corner (from the box, no memory round trip); a candidate the
+            // offsets place near r_l takes the float64 test either way
+            const double fx = (double)cx / nx, fy = (double)cy / ny, fz = (double)cz / nz;
+            for (int c = 0; c < 3; ++c) mref[c] = fx * P.h[c] + fy * P.h[3 + c] + fz * P.h[6 + c];
         }
         __syncthreads();
         const int nt = mpre[27];
@@ -748,17 +763,36 @@ __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
             for (int w = 0; w < W; ++w) bits[w] = 0u;
             int mine = 0;
             if (live && staged) {
+                // branch-free float test of 8 candidates at a time: inside
+                // (below the lower margin) and near-tie bits; the rare near
+                // ties then take the float64 test. The atom itself is the
+                // home cell's (stencil cell 13's) candidate mpre[13] + a: the
+                // stencil cells are distinct, so it appears nowhere else.
+                const int fself = mpre[13] + a;
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     for (int u0 = 0; u0 < 32; u0 += 8) {
                         const int fb = f0 + 32 * w + u0;
                         if (fb >= f1) break;
+                        unsigned in = 0u, near = 0u;
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
+                            const int f = fb + u;
+                            const bool ok = f < f1 && f != fself;
+                            const int fc = f < f1 ? f : fb;   // a staged index either way
+                            const QT ex = xr - qx[fc], ey = yr - qy[fc], ez = zr - qz[fc];
+                            const QT e2 = (ex * ex + ey * ey) + ez * ez;
+                            in |= (ok && e2 < slo2) ? 1u << u : 0u;
+                            near |= (ok && !(e2 < slo2) && !(e2 > shi2)) ? 1u << u : 0u;
+                        }
+                        while (near != 0u) {
+                            const int u = __builtin_ctz(near);
+                            near &= near - 1u;
                             int sj;
                             unsigned short slot;
-                            if (fb + u < f1 && test(fb + u, sj, slot)) bits[w] |= 1u << (u0 + u);
+                            if (test(fb + u, sj, slot)) in |= 1u << u;
                         }
+                        bits[w] |= in << u0;
                     }
                     mine += __builtin_popcount(bits[w]);
                 }
@@ -906,9 +940,16 @@ int build_impl(int R, sim_replica_t* reps, const SimTile* tiles, int ntiles, con
             int* act = trig != nullptr ? trig->act : atom_slot;
             if (trig == nullptr)
                 hipLaunchKernelGGL(k_act_base, dim3(1), dim3(256), 0, st, R, reps, flag, act);
-            hipLaunchKernelGGL((k_list_staged<TC, T, STRIDE>), dim3(SIM_LIST_GRID), dim3(SIM_TILE),
-                               0, st, R, reps, act, x, cell_count, cell_start, nbr, nnbr,
-                               n_stride, max_nbr, overflow, pos_last, ro->nbr16, ro->stencil_max);
+            if (R <= SIM_LIST_ACT_LDS)
+                hipLaunchKernelGGL((k_list_staged<TC, T, STRIDE, true>), dim3(SIM_LIST_GRID),
+                                   dim3(SIM_TILE), 0, st, R, reps, act, x, cell_count, cell_start,
+                                   nbr, nnbr, n_stride, max_nbr, overflow, pos_last, ro->nbr16,
+                                   ro->stencil_max);
+            else
+                hipLaunchKernelGGL((k_list_staged<TC, T, STRIDE, false>), dim3(SIM_LIST_GRID),
+                                   dim3(SIM_TILE), 0, st, R, reps, act, x, cell_count, cell_start,
+                                   nbr, nnbr, n_stride, max_nbr, overflow, pos_last, ro->nbr16,
+                                   ro->stencil_max);
         }
         if (!(staged && ro->stencil3_only))
             hipLaunchKernelGGL((k_list<TC, T, STRIDE, true>), dim3(ntiles), dim3(SIM_TILE), 0,
