@@ -649,6 +649,9 @@ __global__ void __launch_bounds__(SIM_TILE, SIM_LIST_WAVES)
             if ((ACT_LDS ? sact[mid] : act[mid]) <= ac) lo = mid;
             else hi = mid - 1;
         }
+        // equal on every lane (ac is the workgroup's): the replica's fields
+        // then come by scalar loads
+        lo = __builtin_amdgcn_readfirstlane(lo);
         const sim_replica_t& P = reps[lo];
         const int cl = ac - act[lo];
         const int nx = P.ncell[0], ny = P.ncell[1], nz = P.ncell[2];

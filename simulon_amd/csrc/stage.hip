@@ -255,26 +255,30 @@ __global__ void __launch_bounds__(NT)
     __shared__ int cntl[SIM_TILE], wmax[SIM_TILE / 64];
     __shared__ int s_bad, s_live, gate[GateWalk<SIM_STAGE_TILES>::CH];
     GateWalk<SIM_STAGE_TILES> walk(gate, ntiles, tiles, flag, ntiles);
-    for (int tb = walk.next(); tb >= 0; tb = walk.next()) {
+    for (int tb0 = walk.next(); tb0 >= 0; tb0 = walk.next()) {
+        // tb is the workgroup's: the tile and replica records, the group's
+        // count and table pointer come by scalar loads
+        const int tb = __builtin_amdgcn_readfirstlane(tb0);
         const int G = tb / SIM_STAGE_TILES;
+        const int t = threadIdx.x, a = t / LPA, l = t % LPA;
+        const SimTile tile = tiles[tb];
+        const sim_replica_t& P = reps[tile.rep];
+        const int i = tile.start + a;
+        const bool active = i < P.atom_base + P.natoms;
+        const int cnt = active ? nnbr[i] : 0;
+        const int ci = active ? atom_cell[i] : 0;
         __syncthreads();   // the previous tile's LDS reads are done
         // count[G] may turn -1 under another workgroup of the group: one read,
         // shared, keeps the branch uniform
-        if (threadIdx.x == 0) s_live = count[G] >= 0 ? 1 : 0;
+        if (t == 0) s_live = count[G] >= 0 ? 1 : 0;
         __syncthreads();
         if (!s_live) continue;
-        const int t = threadIdx.x, a = t / LPA, l = t % LPA;
         const int* T = table + (size_t)G * 3 * UC;
         for (int h = t; h < UC; h += NT) {
             tkey[h] = T[h];
             toff[h] = T[UC + h];
             tstart[h] = T[2 * UC + h];
         }
-        const SimTile tile = tiles[tb];
-        const sim_replica_t& P = reps[tile.rep];
-        const int i = tile.start + a;
-        const bool active = i < P.atom_base + P.natoms;
-        const int cnt = active ? nnbr[i] : 0;
         if (l == 0) cntl[a] = cnt;
         if (t == 0) s_bad = 0;
         __syncthreads();
@@ -285,21 +289,20 @@ __global__ void __launch_bounds__(NT)
         __syncthreads();
         const int pad_end = min((wmax[a / 64] + 7) & ~7, max_nbr);
         if (active) {
-            const int hs = find<UC>(tkey, atom_cell[i]);
+            const int hs = find<UC>(tkey, ci);
             if (hs < 0) s_bad = 1;
             const int self = hs < 0 ? 0 : toff[hs] + (i - tstart[hs]);
-            // 24 entries per thread in flight: one pass for lists up to 96
-            constexpr int B = 24;
+            constexpr int B = 24;   // rows per thread in flight: one pass for lists up to 96
             for (int k0 = l; k0 < cnt; k0 += LPA * B) {
                 int j[B], c[B];
-    #pragma unroll
+#pragma unroll
                 for (int u = 0; u < B; ++u) {
                     const int k = k0 + LPA * u;
                     j[u] = k < cnt ? nbr[(size_t)k * n + i] : i;
                 }
-    #pragma unroll
+#pragma unroll
                 for (int u = 0; u < B; ++u) c[u] = atom_cell[j[u]];
-    #pragma unroll
+#pragma unroll
                 for (int u = 0; u < B; ++u) {
                     const int k = k0 + LPA * u;
                     if (k < cnt) {
