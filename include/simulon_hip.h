@@ -33,6 +33,8 @@
  *        ("cuda source/neighbor_search_kernel.cu:73-138") and the PyTorch
  *        brute force core/neighbor_search/gpu_kdtree.py:90-156
  *   sim_nl_trigger                 skin/2 test of io_utils/reader.py:136-149
+ *   sim_nl_update                  sim_nl_trigger + sim_nl_build (with a
+ *        reorder) in fewer launches, the same two reference interfaces
  *   sim_lj_force                   simulon_cuda.lj_energy_force_cuda
  *        ("cuda source/lj_energy_force.cpp:10-20") + configure_lj_smoothing
  *        ("cuda source/ops.cpp:39-45") and core/force/lennard_jones_force.py:15-95
@@ -40,8 +42,9 @@
  *        ("cuda source/eam_cuda_ext.cpp:30-60") and the rest of
  *        core/force/eam_force_cu.py:461-579; their _staged variants and
  *        sim_eam_force_second (the force pass with the integrator's second
- *        half) the same passes, with sim_nl_stage building the LDS staging
- *        (no counterpart in the reference)
+ *        half; with four more arguments also the next step's first half)
+ *        the same passes, with sim_nl_stage building the LDS staging (no
+ *        counterpart in the reference)
  *   sim_pos_quantize               Box.wrap_positions / minimum_image
  *        (core/box.py:109-142) in fractional fixed point
  *   sim_integrate_first / _second  core/integrator/integrator.py:91-124
