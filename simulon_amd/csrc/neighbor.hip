@@ -348,7 +348,16 @@ __global__ void k_rank_permute(const sim_replica_t* reps, const SimTile* tiles, 
     const int* a = cell_atoms + c0;
     const int n = cell_count[c];
     int rank = 0;
-    for (int k = 0; k < n; ++k) rank += a[k] < i ? 1 : 0;
+    // the cell's atoms are independent loads: eight in flight at a time
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = a[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rank += v[u] < i ? 1 : 0;
+    }
+    for (; k < n; ++k) rank += a[k] < i ? 1 : 0;
     const int s = c0 + rank;
     double4 x = pos[i];
     const double* hi = P.hinv;
