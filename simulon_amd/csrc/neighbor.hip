@@ -128,15 +128,38 @@ template <typename T, int STRIDE>
 __global__ void k_bin(const sim_replica_t* reps, const SimTile* tiles, const T* x, const int* flag,
                       int* cell_count, int* atom_cell, int* atom_slot) {
     const SimTile t = tiles ? tiles[blockIdx.x] : SimTile{0, (int)blockIdx.x * SIM_TILE};
-    if (!rep_active(flag, t.rep)) return;
+    if (!rep_active(flag, t.rep)) return;   // uniform: the whole tile
     const sim_replica_t& P = reps[t.rep];
-    int i = t.start + threadIdx.x;
-    if (i >= P.atom_base + P.natoms) return;
-    double a, b, c;
-    load_xyz<T, STRIDE>(x, i, a, b, c);
-    int cell = cell_of(P, a, b, c);
-    atom_cell[i] = cell;
-    atom_slot[i] = atomicAdd(&cell_count[cell], 1);
+    const int i = t.start + threadIdx.x;
+    const bool act = i < P.atom_base + P.natoms;
+    const int lane = threadIdx.x & 63;
+    int cell = -1 - lane;   // inactive lanes: a run of their own, no count
+    if (act) {
+        double a, b, c;
+        load_xyz<T, STRIDE>(x, i, a, b, c);
+        cell = cell_of(P, a, b, c);
+        atom_cell[i] = cell;
+    }
+    // Atoms keep the cell order of the last build, so neighbouring lanes
+    // mostly share a cell: one atomic per run of equal cells in the wave
+    // (its last lane) reserves the run's slots, instead of one per atom on
+    // the same counter. rs / re: first and last lane of this lane's run.
+    // (both shuffles run on every lane: a lane that skips one would read 0
+    // from the lanes it skipped)
+    const int up = __shfl_up(cell, 1), down = __shfl_down(cell, 1);
+    const bool start = lane == 0 || up != cell;
+    const bool end = lane == 63 || down != cell;
+    int rs = start ? lane : 0, re = end ? lane : 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(rs, o), d = __shfl_down(re, o);
+        if (lane >= o) rs = max(rs, u);
+        if (lane + o < 64) re = min(re, d);
+    }
+    int base = 0;
+    if (act && end) base = atomicAdd(&cell_count[cell], lane - rs + 1);
+    base = __shfl(base, re);
+    if (act) atom_slot[i] = base + (lane - rs);
 }
 
 // Exclusive scan of cnt[0..n) into start[0..n], one workgroup of 1024
