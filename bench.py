@@ -212,7 +212,11 @@ def eam_kernel_roofline(eng, workload="w_eam_replicas", virial=False):
 
 def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
     """Roofline of the LJ force kernel the step launches (with the virial
-    only when the step computes it: NPT); bytes per atom 12 + 4 nbar + 12."""
+    only when the step computes it: NPT); bytes per atom 12 + 4 nbar + 12,
+    + 24 (v_half read, v write) for the global-list pass, which does the
+    step's second half too (sim_lj_force_second). Repeating it is idempotent:
+    v is rewritten from the same v_half, forces and noise (the step counter
+    does not advance)."""
     from simulon_amd._lib import call, ptr, stream_ptr
     ff = eng.force_field
     nbar = mean_full_list(eng)
@@ -224,12 +228,12 @@ def lj_kernel_roofline(eng, workload="lj_argon_npt_tri", virial=True):
         eng.quantize()
 
         def k():
-            call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-                 ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force), ptr(eng.partial),
-                 int(virial), stream_ptr())
+            call("sim_lj_force_second", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles,
+                 ptr(eng.posq), ptr(eng.nbr), ptr(eng.nnbr), eng.N, ff.params, ptr(eng.force),
+                 ptr(eng.partial), int(virial), *eng.second_args(), stream_ptr())
 
     t = kernel_time_ms(k)
-    b = eng.N * (12 + 4 * nbar + 12)
+    b = eng.N * (12 + 4 * nbar + 12 + (0 if eng.use_tiled else 24))
     ach = b / (t * 1e-3) / 1e9
     name = "lj_force" + (f"_{eng.family}" if eng.use_tiled else "")
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,

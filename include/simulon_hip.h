@@ -316,6 +316,19 @@ int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
                  sim_lj_params_t lj, float* force, double* partial, int virial,
                  void* stream);
 
+/* sim_lj_force with the step's second half fused in: after an atom's force
+ * the pass does sim_integrate_second's O + B for it (the same bits) and
+ * writes the kinetic columns SIM_T_KE..+6 of the tile's partial row, so no
+ * second-half launch follows (core/force/lennard_jones_force.py:15-95 and
+ * core/integrator/integrator.py:102-124). Arguments after virial as
+ * sim_integrate_second's. */
+int sim_lj_force_second(int R, const void* reps, const int* tiles, int ntiles,
+                        const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                        sim_lj_params_t lj, float* force, double* partial, int virial,
+                        sim_types_t types, float* vel_half, float* vel, const float* noise,
+                        const long long* key, unsigned long long seed,
+                        const unsigned long long* step_dev, void* stream);
+
 /* EAM (core/force/eam_force_cu.py:461-579). Pass 1: rho_i (written to rho[N]
  * when rho is not NULL), F(rho_i), F'(rho_i) (written to fp[N] when fp is not
  * NULL) and the energy column; rec (int[N][4], not aliasing posq) receives the

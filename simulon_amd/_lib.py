@@ -118,6 +118,8 @@ SIGNATURES = {
     "sim_nl_update": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp, _vp, _I, _vp, _I, _vp, _vp,
                            _vp, _vp, _vp, _vp, _vp, _I, _vp, _I, _vp, _vp, _vp]),
     "sim_lj_force": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, LJParams, _vp, _vp, _I, _vp]),
+    "sim_lj_force_second": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, LJParams, _vp, _vp, _I,
+                                 Types, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
     "sim_pos_quantize": (_I, [_vp, _vp, _I, _vp, _vp, _vp]),
     "sim_eam_density": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _I, EAMParams, _vp, _vp, _vp, _vp,
                              _vp]),

@@ -250,3 +250,25 @@ static inline float sqrt_threshold(float rc) {
 #ifndef SIM_LIST_BATCH
 #define SIM_LIST_BATCH 16
 #endif
+
+// The integrator's second half fused into a force pass (INTEG; the EAM force
+// pass and k_lj): with the force of atom i in registers the pass finishes the
+// step for it (O + B, the arithmetic of k_second) and writes the kinetic
+// columns of its tile's partial row, instead of a separate kernel re-reading
+// forces and v_half. With pos set (EAM), the pass also does the next step's
+// first half for the replicas first_fusable admits (force and new velocity
+// in registers): v_half, position, record, and the displacement maximum.
+struct SecondHalf {
+    float* vh;
+    float* vel;
+    const float* noise;
+    const long long* key;
+    unsigned long long seed;
+    const unsigned long long* step_dev;
+    sim_types_t ty;
+    double4* pos;
+    const double4* pos_last;
+    unsigned long long* maxd2;
+    int4* posq;
+};
+

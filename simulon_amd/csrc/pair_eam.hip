@@ -530,27 +530,6 @@ __device__ __forceinline__ void force_atom_staged(ForceAcc& a, int i, const int4
     }
 }
 
-// The integrator's second half fused into the force pass (INTEG): with the
-// force of atom i in registers the pass finishes the step for it (O + B, the
-// arithmetic of k_second) and writes the kinetic columns of its tile's
-// partial row, instead of a separate kernel re-reading forces and v_half.
-// With pos set, the pass also does the next step's first half for the
-// replicas first_fusable admits (force and new velocity in registers):
-// v_half, position, record, and the displacement maximum for the trigger.
-struct SecondHalf {
-    float* vh;
-    float* vel;
-    const float* noise;
-    const long long* key;
-    unsigned long long seed;
-    const unsigned long long* step_dev;
-    sim_types_t ty;
-    double4* pos;
-    const double4* pos_last;
-    unsigned long long* maxd2;
-    int4* posq;
-};
-
 template <int NT, bool LT, bool VIRIAL, int TPW, bool INTEG, bool STG>
 __global__ void __launch_bounds__(SIM_TILE * TPW, LT ? SIM_EAM_FORCE_WAVES : 1)
     k_eam_force(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,

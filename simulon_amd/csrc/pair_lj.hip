@@ -154,13 +154,16 @@ __device__ __forceinline__ void lj_atom(LJAcc& a, int i, const int4 xi, int cnt,
 #define SIM_LJ_TPW 1
 #endif
 
-template <bool VIRIAL, int MODE, bool NT1>
+// INTEG: the step's second half fused in (S; the arithmetic of k_second,
+// kinetic terms into columns SIM_T_KE.. of the tile's partial row).
+template <bool VIRIAL, int MODE, bool NT1, bool INTEG>
 __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
     k_lj(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles, int ntiles,
          const int4* __restrict__ pos, const int* __restrict__ nbr,
          const int* __restrict__ nnbr, int n_stride, sim_lj_params_t lj,
-         float* __restrict__ force, double* __restrict__ partial, float thr) {
-    __shared__ double lds[SIM_LJ_TPW * 4 * 8];
+         float* __restrict__ force, double* __restrict__ partial, float thr, SecondHalf S) {
+    constexpr int NV = INTEG ? 8 + 7 : 8;
+    __shared__ double lds[SIM_LJ_TPW * 4 * NV];
     const int g = threadIdx.x / SIM_TILE, lt = threadIdx.x % SIM_TILE;
     const int tb = xcd_block(blockIdx.x, gridDim.x) * SIM_LJ_TPW + g;
     const bool have = tb < ntiles;
@@ -171,6 +174,14 @@ __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
     LJAcc a;
     const int cnt = active ? nnbr[i] : 0;
     const int cmax = wave_max_int(cnt), cmin = wave_min_int(active ? cnt : INT_MAX);
+    // the second half's O step needs no force: before the walk
+    float3 w_o = make_float3(0.f, 0.f, 0.f);
+    int ty_i = 0;
+    if (INTEG && active) {
+        const unsigned long long step = S.step_dev ? S.step_dev[0] : 0ull;
+        ty_i = S.ty.ntypes == 1 ? 0 : pos[i].w;
+        w_o = second_half_pre(P, i, ty_i, S.vh, S.noise, S.key, S.seed, step);
+    }
     if (active) {
         const QBoxD B = load_qbox_d(P);
         const int4 xi = pos[i];
@@ -185,38 +196,55 @@ __global__ void __launch_bounds__(SIM_TILE * SIM_LJ_TPW)
         force[(size_t)i * 3 + 1] = a.fy;
         force[(size_t)i * 3 + 2] = a.fz;
     }
-    double v[8] = {0.5 * a.e, 0.5 * (a.wxx + a.wyy + a.wzz), 0.5 * a.wxx, 0.5 * a.wyy,
-                   0.5 * a.wzz, 0.5 * a.wxy, 0.5 * a.wxz, 0.5 * a.wyz};
-    group_sum<8>(v, lds, g, lt);
+    double v[NV];
+    v[0] = 0.5 * a.e;
+    v[1] = 0.5 * (a.wxx + a.wyy + a.wzz);
+    v[2] = 0.5 * a.wxx;
+    v[3] = 0.5 * a.wyy;
+    v[4] = 0.5 * a.wzz;
+    v[5] = 0.5 * a.wxy;
+    v[6] = 0.5 * a.wxz;
+    v[7] = 0.5 * a.wyz;
+    if (INTEG) {
+        double kin[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            const float m_i = S.ty.ntypes == 1 ? S.ty.mass[0] : S.ty.mass[ty_i];
+            (void)second_half_post(P, i, m_i, w_o, a.fx, a.fy, a.fz, S.vel, kin);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v[8 + k] = kin[k];
+    }
+    group_sum<NV>(v, lds, g, lt);
     if (have && lt == 0) {
         double* p = partial + (size_t)tb * SIM_NPART;
 #pragma unroll
         for (int k = 0; k < 8; ++k) p[k] = v[k];
+        if (INTEG) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) p[SIM_T_KE + k] = v[8 + k];
+        }
     }
 }
 
-template <bool V, bool NT1>
+template <bool V, bool NT1, bool I>
 void launch_mode(int mode, dim3 g, hipStream_t st, const sim_replica_t* reps, const SimTile* tiles,
                  int ntiles, const int4* pos, const int* nbr, const int* nnbr, int ns,
-                 sim_lj_params_t lj, float* f, double* part, float thr) {
+                 sim_lj_params_t lj, float* f, double* part, float thr, const SecondHalf& S) {
     const dim3 b(SIM_TILE * SIM_LJ_TPW);
     if (mode == 1)
-        hipLaunchKernelGGL((k_lj<V, 1, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 1, NT1, I>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr, S);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_lj<V, 2, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 2, NT1, I>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr, S);
     else
-        hipLaunchKernelGGL((k_lj<V, 0, NT1>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr, ns,
-                           lj, f, part, thr);
+        hipLaunchKernelGGL((k_lj<V, 0, NT1, I>), g, b, 0, st, reps, tiles, ntiles, pos, nbr, nnbr,
+                           ns, lj, f, part, thr, S);
 }
 
-}  // namespace
-
-extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
-                            const int* posq, const int* nbr, const int* nnbr, int n_stride,
-                            sim_lj_params_t lj, float* force, double* partial, int virial,
-                            void* stream) {
+int lj_impl(int R, const void* reps, const int* tiles, int ntiles, const int* posq, const int* nbr,
+            const int* nnbr, int n_stride, sim_lj_params_t lj, float* force, double* partial,
+            int virial, const SecondHalf* S2, void* stream) {
     if (R <= 0 || !reps || !tiles || ntiles <= 0 || !posq || !nbr || !nnbr || n_stride <= 0 ||
         n_stride >= (1 << 28) ||
         !force || !partial || lj.mode < 0 || lj.mode > 2 || !(lj.rc > 0.f) || lj.ntypes < 1 ||
@@ -228,22 +256,50 @@ extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntile
     auto* X = reinterpret_cast<const int4*>(posq);
     const dim3 g((ntiles + SIM_LJ_TPW - 1) / SIM_LJ_TPW);
     hipStream_t st = as_stream(stream);
-    auto go = [&](auto v, auto one) {
-        launch_mode<decltype(v)::value, decltype(one)::value>(lj.mode, g, st, P, T, ntiles, X, nbr,
-                                                              nnbr, n_stride, lj, force, partial,
-                                                              thr);
+    const SecondHalf S = S2 ? *S2 : SecondHalf{};
+    auto go = [&](auto v, auto one, auto in) {
+        launch_mode<decltype(v)::value, decltype(one)::value, decltype(in)::value>(
+            lj.mode, g, st, P, T, ntiles, X, nbr, nnbr, n_stride, lj, force, partial, thr, S);
     };
     using yes = std::true_type;
     using no = std::false_type;
-    if (lj.ntypes == 1) {
-        if (virial) go(yes(), yes());
-        else go(no(), yes());
-    } else {
-        if (virial) go(yes(), no());
-        else go(no(), no());
-    }
+    auto go2 = [&](auto in) {
+        if (lj.ntypes == 1) {
+            if (virial) go(yes(), yes(), in);
+            else go(no(), yes(), in);
+        } else {
+            if (virial) go(yes(), no(), in);
+            else go(no(), no(), in);
+        }
+    };
+    if (S2) go2(yes());
+    else go2(no());
     SIM_CHECK_LAUNCH();
     return SIM_OK;
+}
+
+}  // namespace
+
+extern "C" int sim_lj_force(int R, const void* reps, const int* tiles, int ntiles,
+                            const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                            sim_lj_params_t lj, float* force, double* partial, int virial,
+                            void* stream) {
+    return lj_impl(R, reps, tiles, ntiles, posq, nbr, nnbr, n_stride, lj, force, partial, virial,
+                   nullptr, stream);
+}
+
+extern "C" int sim_lj_force_second(int R, const void* reps, const int* tiles, int ntiles,
+                                   const int* posq, const int* nbr, const int* nnbr, int n_stride,
+                                   sim_lj_params_t lj, float* force, double* partial, int virial,
+                                   sim_types_t types, float* vel_half, float* vel,
+                                   const float* noise, const long long* key,
+                                   unsigned long long seed, const unsigned long long* step_dev,
+                                   void* stream) {
+    if (!vel_half || !vel || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES) return SIM_E_ARG;
+    const SecondHalf S{vel_half, vel, noise, key, seed, step_dev, types,
+                       nullptr, nullptr, nullptr, nullptr};
+    return lj_impl(R, reps, tiles, ntiles, posq, nbr, nnbr, n_stride, lj, force, partial, virial,
+                   &S, stream);
 }
 
 extern "C" int sim_version(char* buf, int len) {

@@ -40,22 +40,32 @@ class LJForceField:
         self.ntypes = nt
         self.rc = float(rc)
 
-    def compute(self, eng, virial=True):
+    # the global-list pass can do the step's second half (sim_lj_force_second)
+    fuses_second = True
+
+    def compute(self, eng, virial=True, second=None):
+        """second: the engine's second_args(); the global-list pass then also
+        does the step's second half. Returns whether it did."""
         if len(eng.masses_amu) > self.ntypes:
             raise ValueError(f"{len(eng.masses_amu)} atom types, LJ tables for {self.ntypes}")
         if eng.use_tiled and eng.family == "rows":
             call("sim_lj_force_rows", *eng.row_args(), len(eng.masses_amu), self.params,
                  ptr(eng.force), ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow),
                  stream_ptr())
-            return
+            return False
         if eng.use_tiled:
             call("sim_lj_force_cell", *eng.cell_args(), self.params, ptr(eng.force),
                  ptr(eng.partial_cell), int(bool(virial)), ptr(eng.overflow), stream_ptr())
-            return
+            return False
         eng.quantize()
-        call("sim_lj_force", eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq),
-             ptr(eng.nbr), ptr(eng.nnbr), eng.N, self.params, ptr(eng.force), ptr(eng.partial),
-             int(bool(virial)), stream_ptr())
+        args = (eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq), ptr(eng.nbr),
+                ptr(eng.nnbr), eng.N, self.params, ptr(eng.force), ptr(eng.partial),
+                int(bool(virial)))
+        if second is not None and self.fuses_second:
+            call("sim_lj_force_second", *args, *second, stream_ptr())
+            return True
+        call("sim_lj_force", *args, stream_ptr())
+        return False
 
 
 class EAMForceField:

@@ -7,7 +7,9 @@ global tables (small launch) and on the persistent LDS-table kernel (more
 than 256 tiles). Graph runs of two or more steps also let the force pass do
 the next step's first half (the NPT replica's stays in its own kernel): the
 same bits as runs that do not, across list rebuilds and runs of 1, 2 and more
-steps.
+steps. The LJ global-list pass with the second half fused in
+(sim_lj_force_second) equals k_lj followed by the separate kernel, bit for
+bit, on triclinic argon cells packed the same way.
 """
 import numpy as np
 import pytest
@@ -15,7 +17,7 @@ import torch
 
 from simulon_amd import lattice as lat
 from simulon_amd.engine import ReplicaEngine, ReplicaSpec
-from simulon_amd.forcefield import EAMForceField
+from simulon_amd.forcefield import EAMForceField, LJForceField
 
 pytestmark = pytest.mark.gpu
 
@@ -86,3 +88,37 @@ def test_next_first_half_chain_bitwise(setfl, sides):
         x, y = getattr(a, name), getattr(b, name)
         assert torch.equal(x, y), f"{name} differs"
     np.testing.assert_array_equal(a.thermo_host(), b.thermo_host())
+
+
+def _lj_engine(fused, sides=(6, 5, 7)):
+    kinds = [dict(ensemble="NVT", temperature=120.0, gamma=5.0),
+             dict(ensemble="NVE"),
+             dict(ensemble="NPT", temperature=90.0, target_pressure=1.0, barostat="berendsen",
+                  tau_p=0.5)]
+    specs = []
+    for k, n in enumerate(sides):
+        pos, H = lat.lj_argon_npt_tri(ncell=n, seed=21 + k)
+        specs.append(ReplicaSpec(positions=pos, box=H, dt=0.002, **kinds[k % 3]))
+    eng = ReplicaEngine(specs, [lat.AR_MASS], cutoff=8.5, skin=1.0, seed=17, family="global")
+    ff = LJForceField([[lat.AR_EPS]], [[lat.AR_SIG]], 8.5)
+    if not fused:
+        ff.fuses_second = False            # instance attribute: the separate kernel
+    eng.force_field = ff
+    eng.setup()
+    return eng
+
+
+def test_lj_fused_second_half_bitwise():
+    a = _lj_engine(True)
+    b = _lj_engine(False)
+    print(f"[case] LJ fused second half: {a.N} atoms, {a.ntiles} tiles", flush=True)
+    for eng in (a, b):
+        eng.run(25, graph=True)
+        eng.run(3, graph=False)
+    torch.cuda.synchronize()
+    for name in ("pos4", "vel", "vel_half", "force"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert torch.equal(x, y), f"{name}: max diff {(x - y).abs().max().item():.3e}"
+    ta, tb = a.thermo_host(), b.thermo_host()
+    np.testing.assert_array_equal(ta, tb)
+    assert np.all(np.isfinite(ta))
