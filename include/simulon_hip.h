@@ -35,7 +35,7 @@
  *   sim_nl_trigger                 skin/2 test of io_utils/reader.py:136-149
  *   sim_nl_update                  sim_nl_trigger + sim_nl_build (with a
  *        reorder) in fewer launches, the same two reference interfaces
- *   sim_lj_force                   simulon_cuda.lj_energy_force_cuda
+ *   sim_lj_force (/ _second)       simulon_cuda.lj_energy_force_cuda
  *        ("cuda source/lj_energy_force.cpp:10-20") + configure_lj_smoothing
  *        ("cuda source/ops.cpp:39-45") and core/force/lennard_jones_force.py:15-95
  *   sim_eam_density / sim_eam_force   simulon_cuda.density_pass / force_pass
