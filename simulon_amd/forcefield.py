@@ -153,7 +153,11 @@ class EAMForceField:
         rec = eng.scratch("eam_rec", torch.int32, (eng.N, 4))
         eng.quantize()
         rho = eng.scratch("eam_rho", torch.float32) if keep_rho else None
-        st = eng.stage_arrays() if hasattr(eng, "stage_arrays") else None
+        # LDS staging only for the engine's own force field: its steps rebuild
+        # the stage with the lists. A call into work arrays (an EAM judge of
+        # another force field) reads the lists directly: a stage it created
+        # would not be rebuilt by that engine's captured steps and go stale.
+        st = eng.stage_arrays() if own and hasattr(eng, "stage_arrays") else None
         dargs = (eng.R, ptr(eng.reps), ptr(eng.tiles), eng.ntiles, ptr(eng.posq), ptr(eng.nbr),
                  ptr(eng.nnbr), eng.N, self.params, ptr(rho), None, ptr(rec), ptr(partial))
         if st is not None:
