@@ -465,12 +465,18 @@ int sim_integrate_first(const void* reps, const int* tiles, int ntiles,
 /* sim_integrate_first that also writes the position records of the new
  * positions (as sim_pos_quantize) into posq [N][4]: a step whose list build
  * passes the same posq in sim_nl_reorder_t needs no sim_pos_quantize.
- * unfused_only = 1: only the replicas with a barostat or loading (the others'
- * first half was done by the previous step's sim_eam_force_second). */
+ * flags: SIM_FIRST_UNFUSED_ONLY: only the replicas with a barostat or
+ * loading (the others' first half was done by the previous step's
+ * sim_eam_force_second); SIM_FIRST_BERENDSEN_SCALE: for the Berendsen
+ * replicas, first the scaling of x, pos_last and vel by mu that the previous
+ * step's sim_berendsen_box left undone (sim_berendsen's arithmetic; vel and
+ * pos_last are then written). */
+#define SIM_FIRST_UNFUSED_ONLY 1
+#define SIM_FIRST_BERENDSEN_SCALE 2
 int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
-                          sim_types_t types, double* pos4, const float* vel,
-                          const float* force, float* vel_half, const double* pos_last,
-                          double* maxd2, int* posq, int unfused_only, void* stream);
+                          sim_types_t types, double* pos4, float* vel,
+                          const float* force, float* vel_half, double* pos_last,
+                          double* maxd2, int* posq, int flags, void* stream);
 
 /* Second half (O + B): v = vh damp + xi rff (Langevin replicas), then
  * v += 1/2 F/m dt; KE and kinetic-tensor columns of partial.
@@ -500,6 +506,13 @@ int sim_reduce(int R, const int* rep_rows, const double* partial, int col0, int 
  * read the counter, saves the step a launch of its own. */
 int sim_reduce_step(int R, const int* rep_rows, const double* partial, int col0, int col1,
                     double* chunks, double* thermo, unsigned long long* step_dev, void* stream);
+
+/* The box half of sim_berendsen (pressure, mu, H, volume, the trigger's box
+ * bound) without scaling the atoms: a caller that runs several steps back to
+ * back can leave that scaling to the next step's sim_integrate_first_q with
+ * SIM_FIRST_BERENDSEN_SCALE (the same bits); the last step of a run calls
+ * sim_berendsen. core/barostat.py:36-68. */
+int sim_berendsen_box(int R, void* reps, const double* thermo, void* stream);
 
 /* Berendsen (barostat.py:36-68) for replicas with barostat = 1: pressure from
  * thermo K + W and the volume, mu, box update in reps; then x, v and

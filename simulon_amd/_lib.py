@@ -21,6 +21,7 @@ MLP_NDESC, MLP_NH, MLP_ROWS_PER_TILE = 32, 64, 4
 MLP_NPARAM = 6337
 MLP_NGRAD = MLP_NPARAM + 1
 T_EPOT, T_VIRIAL, T_W, T_KE, T_K = 0, 1, 2, 8, 9
+SIM_FIRST_UNFUSED_ONLY, SIM_FIRST_BERENDSEN_SCALE = 1, 2
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
@@ -145,6 +146,7 @@ SIGNATURES = {
     "sim_reduce": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp]),
     "sim_reduce_step": (_I, [_I, _vp, _vp, _I, _I, _vp, _vp, _vp, _vp]),
     "sim_berendsen": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _vp]),
+    "sim_berendsen_box": (_I, [_I, _vp, _vp, _vp]),
     "sim_aniso_barostat": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp, _vp, _U64, _vp, _vp]),
     "sim_box_deform": (_I, [_I, _vp, _vp, _I, _vp, _vp, _vp]),
     "sim_pos_export": (_I, [_vp, _I, _vp, _I, _vp]),

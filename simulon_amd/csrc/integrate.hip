@@ -20,22 +20,43 @@ __global__ void __launch_bounds__(SIM_TILE)
     k_first(const sim_replica_t* __restrict__ reps, const SimTile* __restrict__ tiles,
             sim_types_t ty, double4* __restrict__ pos, const float* __restrict__ vel,
             const float* __restrict__ force, float* __restrict__ vh_out,
-            const double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2,
-            int4* __restrict__ posq, int unfused_only) {
+            double4* __restrict__ pos_last, unsigned long long* __restrict__ maxd2,
+            int4* __restrict__ posq, int flags, float* __restrict__ vel_w) {
     __shared__ double lds[4];
     const SimTile t = tile_of(tiles);
     const sim_replica_t& P = reps[t.rep];
     // the previous step's EAM force pass did this replica's first half
-    if (unfused_only && first_fusable(P)) return;   // uniform over the tile
+    if ((flags & SIM_FIRST_UNFUSED_ONLY) && first_fusable(P)) return;   // uniform over the tile
+    // the previous step's Berendsen update left its scaling of x, the build
+    // positions and v to this launch (k_baro_scale's arithmetic, then B + A)
+    const bool scale = (flags & SIM_FIRST_BERENDSEN_SCALE) && P.barostat == 1;
     const int i = t.start + threadIdx.x;
     double d2 = 0.0;
     if (i < P.atom_base + P.natoms) {
         double4 x = pos[i];
-        const float m = ty.mass[(int)x.w];
+        double4 l = pos_last[i];
         const size_t o = (size_t)i * 3;
+        float3 v = make_float3(vel[o], vel[o + 1], vel[o + 2]);
+        if (scale) {
+            const double mu = P.mu;
+            const float muf = (float)mu;
+            x.x *= mu;
+            x.y *= mu;
+            x.z *= mu;
+            l.x *= mu;
+            l.y *= mu;
+            l.z *= mu;
+            pos_last[i] = l;
+            v.x *= muf;
+            v.y *= muf;
+            v.z *= muf;
+            vel_w[o] = v.x;
+            vel_w[o + 1] = v.y;
+            vel_w[o + 2] = v.z;
+        }
+        const float m = ty.mass[(int)x.w];
         float3 h;
-        d2 = first_half_atom(P, m, make_float3(vel[o], vel[o + 1], vel[o + 2]), force[o],
-                             force[o + 1], force[o + 2], x, pos_last[i], h);
+        d2 = first_half_atom(P, m, v, force[o], force[o + 1], force[o + 2], x, l, h);
         vh_out[o] = h.x;
         vh_out[o + 1] = h.y;
         vh_out[o + 2] = h.z;
@@ -434,19 +455,20 @@ extern "C" int sim_pos_quantize(const void* reps, const int* tiles, int ntiles, 
 }
 
 static int first_impl(const void* reps, const int* tiles, int ntiles, sim_types_t types,
-                      double* pos4, const float* vel, const float* force, float* vel_half,
-                      const double* pos_last, double* maxd2, int* posq, int unfused_only,
+                      double* pos4, float* vel, const float* force, float* vel_half,
+                      double* pos_last, double* maxd2, int* posq, int flags,
                       void* stream) {
     if (!reps || !tiles || ntiles <= 0 || !pos4 || !vel || !force || !vel_half || !pos_last ||
-        !maxd2 || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES)
+        !maxd2 || types.ntypes < 1 || types.ntypes > SIM_MAX_TYPES ||
+        (flags & ~(SIM_FIRST_UNFUSED_ONLY | SIM_FIRST_BERENDSEN_SCALE)) != 0)
         return SIM_E_ARG;
     hipLaunchKernelGGL(k_first, dim3(ntiles), dim3(SIM_TILE), 0, as_stream(stream),
                        static_cast<const sim_replica_t*>(reps),
                        reinterpret_cast<const SimTile*>(tiles), types,
                        reinterpret_cast<double4*>(pos4), vel, force, vel_half,
-                       reinterpret_cast<const double4*>(pos_last),
+                       reinterpret_cast<double4*>(pos_last),
                        reinterpret_cast<unsigned long long*>(maxd2), reinterpret_cast<int4*>(posq),
-                       unfused_only);
+                       flags, vel);
     SIM_CHECK_LAUNCH();
     return SIM_OK;
 }
@@ -455,18 +477,18 @@ extern "C" int sim_integrate_first(const void* reps, const int* tiles, int ntile
                                    sim_types_t types, double* pos4, const float* vel,
                                    const float* force, float* vel_half, const double* pos_last,
                                    double* maxd2, void* stream) {
-    return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
-                      nullptr, 0, stream);
+    // flags 0: vel and pos_last are only read
+    return first_impl(reps, tiles, ntiles, types, pos4, const_cast<float*>(vel), force, vel_half,
+                      const_cast<double*>(pos_last), maxd2, nullptr, 0, stream);
 }
 
 extern "C" int sim_integrate_first_q(const void* reps, const int* tiles, int ntiles,
-                                     sim_types_t types, double* pos4, const float* vel,
-                                     const float* force, float* vel_half, const double* pos_last,
-                                     double* maxd2, int* posq, int unfused_only,
-                                     void* stream) {
+                                     sim_types_t types, double* pos4, float* vel,
+                                     const float* force, float* vel_half, double* pos_last,
+                                     double* maxd2, int* posq, int flags, void* stream) {
     if (!posq) return SIM_E_ARG;
     return first_impl(reps, tiles, ntiles, types, pos4, vel, force, vel_half, pos_last, maxd2,
-                      posq, unfused_only, stream);
+                      posq, flags, stream);
 }
 
 extern "C" int sim_integrate_second(const void* reps, const int* tiles, int ntiles,
@@ -534,6 +556,14 @@ extern "C" int sim_reduce_step(int R, const int* rep_rows, const double* partial
                                unsigned long long* step_dev, void* stream) {
     if (!step_dev) return SIM_E_ARG;
     return reduce_impl(R, rep_rows, partial, col0, col1, chunks, thermo, step_dev, stream);
+}
+
+extern "C" int sim_berendsen_box(int R, void* reps, const double* thermo, void* stream) {
+    if (R <= 0 || !reps || !thermo) return SIM_E_ARG;
+    hipLaunchKernelGGL(k_baro_box, dim3((R + 63) / 64), dim3(64), 0, as_stream(stream), R,
+                       static_cast<sim_replica_t*>(reps), thermo);
+    SIM_CHECK_LAUNCH();
+    return SIM_OK;
 }
 
 extern "C" int sim_berendsen(int R, void* reps, const int* tiles, int ntiles, const double* thermo,

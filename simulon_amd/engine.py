@@ -170,6 +170,7 @@ class ReplicaEngine:
         self._graph = None
         self._graph_virial = None
         self._fuse_next = False
+        self.chain = True          # graph runs of >= 2 steps may chain (run())
         self.steps_done = 0
         self.loading = False        # a tensile loader deforms the box at the start of each step
         # True while no box has changed since construction (cleared by every
@@ -632,18 +633,22 @@ class ReplicaEngine:
             raise RuntimeError("no force field attached")
         self.force_field.compute(self, virial)
 
-    def first_half(self, records=False, unfused_only=False):
+    def first_half(self, records=False, unfused_only=False, scale=False):
         """B+A; records=True also writes the position records (posq).
         unfused_only: only the replicas whose first half the previous step's
-        force pass did not do (those with a barostat or loading)."""
+        force pass did not do (those with a barostat or loading). scale: the
+        Berendsen replicas first get the scaling of x, v and the build
+        positions that the previous step's barostat left undone."""
         if records:
+            flags = ((_lib.SIM_FIRST_UNFUSED_ONLY if unfused_only else 0)
+                     | (_lib.SIM_FIRST_BERENDSEN_SCALE if scale else 0))
             call("sim_integrate_first_q", ptr(self.reps), ptr(self.tiles), self.ntiles,
                  self.types_struct, ptr(self.pos4), ptr(self.vel), ptr(self.force),
                  ptr(self.vel_half), ptr(self.pos_last), ptr(self.maxd2), ptr(self.posq),
-                 int(bool(unfused_only)), stream_ptr())
+                 flags, stream_ptr())
             return
-        if unfused_only:
-            raise ValueError("unfused_only needs records=True")
+        if unfused_only or scale:
+            raise ValueError("unfused_only and scale need records=True")
         call("sim_integrate_first", ptr(self.reps), ptr(self.tiles), self.ntiles, self.types_struct,
              ptr(self.pos4), ptr(self.vel), ptr(self.force), ptr(self.vel_half),
              ptr(self.pos_last), ptr(self.maxd2), stream_ptr())
@@ -696,9 +701,13 @@ class ReplicaEngine:
             return self.partial_cell, self.rep_cells
         return self.partial, self.rep_tiles
 
-    def barostat(self):
+    def barostat(self, defer_scale=False):
+        """The barostat update of the NPT replicas. defer_scale: the Berendsen
+        replicas' box only; the next step's first half scales their atoms."""
         kinds = {s.barostat for s in self.specs if s.ensemble.upper() == "NPT"}
-        if "berendsen" in kinds:
+        if "berendsen" in kinds and defer_scale:
+            call("sim_berendsen_box", self.R, ptr(self.reps), ptr(self.thermo), stream_ptr())
+        elif "berendsen" in kinds:
             call("sim_berendsen", self.R, ptr(self.reps), ptr(self.tiles), self.ntiles,
                  ptr(self.thermo), ptr(self.pos4), ptr(self.vel), ptr(self.pos_last), stream_ptr())
         if "anisotropic" in kinds:
@@ -743,18 +752,21 @@ class ReplicaEngine:
         """One MD step for every replica (md_model.py:103-153)."""
         self._step(virial, noise)
 
-    def _step(self, virial=None, noise=None, first_unfused=False, fuse_next=False):
+    def _step(self, virial=None, noise=None, first_unfused=False, fuse_next=False,
+              scale_pending=False, defer_scale=False):
         """step() as recorded in the graph replays of run(): first_unfused: the
         previous step's force pass did the first half of the replicas without
         barostat or loading (no first-half launch at all when every replica is
         such); fuse_next: this step's force pass does the next step's first
-        half for them."""
+        half for them; scale_pending: the previous step left its Berendsen
+        scaling of the atoms to this step's first half; defer_scale: this
+        step leaves it to the next."""
         if virial is None:
             virial = True
         if self.loading:
             self.deform()
         if not (first_unfused and self._all_fusable()):
-            self.first_half(records=True, unfused_only=first_unfused)
+            self.first_half(records=True, unfused_only=first_unfused, scale=scale_pending)
         self._fuse_next = fuse_next
         try:
             self.build_and_forces(virial or self.has_barostat, noise, second=True)
@@ -762,7 +774,7 @@ class ReplicaEngine:
             self._fuse_next = False
         self.reduce(0, 15, advance=True)
         if self.has_barostat:
-            self.barostat()
+            self.barostat(defer_scale=defer_scale)
         self.steps_done += 1
 
     # graph replays: _graph (an unfused step) and, when the force field
@@ -779,13 +791,15 @@ class ReplicaEngine:
             self.__dict__["_graphs_chain"] = {}
             self.__dict__["_fusable_all"] = None
 
-    def _record(self, virial, first_unfused=False, fuse_next=False):
+    def _record(self, virial, first_unfused=False, fuse_next=False, scale_pending=False,
+                defer_scale=False):
         if getattr(self, "_stale_slots", False) and self.use_tiled:
             self.build_list(force=True)     # refresh the LDS families' slots first
         g = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
-            self._step(virial, first_unfused=first_unfused, fuse_next=fuse_next)
+            self._step(virial, first_unfused=first_unfused, fuse_next=fuse_next,
+                       scale_pending=scale_pending, defer_scale=defer_scale)
         self.steps_done -= 1
         torch.cuda.synchronize()
         return g
@@ -810,25 +824,38 @@ class ReplicaEngine:
             self.__dict__["_fusable_all"] = v
         return v
 
+    def _defers_scale(self):
+        """Whether a run may leave each Berendsen step's scaling of the atoms
+        to the next step's first half (Berendsen replicas, no loading: a
+        deformation between the two would see unscaled atoms)."""
+        return (not self.loading and any(s.ensemble.upper() == "NPT" and s.barostat == "berendsen"
+                                         for s in self.specs))
+
     def _chain(self, virial):
         """The three graphs of a run of >= 2 steps whose force passes do the
-        next step's first half: (first, middle, last)."""
+        next step's first half and / or whose Berendsen steps leave the
+        scaling of the atoms to the next step's first half: (first, middle,
+        last); the last step does both of its own."""
         chains = self.__dict__.setdefault("_graphs_chain", {})
         if virial not in chains:
             self._all_fusable()        # host read of the records, before recording
-            chains[virial] = (self._record(virial, False, True), self._record(virial, True, True),
-                              self._record(virial, True, False))
+            f, d = self.fuses_next_first(), self._defers_scale()
+            chains[virial] = (self._record(virial, False, f, False, d),
+                              self._record(virial, f, f, d, d),
+                              self._record(virial, f, False, d, False))
         return chains[virial]
 
     def run(self, nsteps, graph=True, virial=True, check=True):
-        """nsteps steps. graph=True replays captured steps; with the EAM
-        global-list pass, a run of two or more steps lets each force pass but
-        the last do the next step's first half (the same bits as separate
-        steps; the state between runs is that of whole steps)."""
+        """nsteps steps. graph=True replays captured steps; a run of two or
+        more steps chains them where it can: with the EAM global-list pass each
+        force pass but the last does the next step's first half, and each
+        Berendsen step but the last leaves its scaling of the atoms to the next
+        step's first half (the same bits as separate steps; the state between
+        runs is that of whole steps; self.chain = False turns chaining off)."""
         if graph:
             if self._graph is None or self._graph_virial != virial:
                 self.capture(virial)
-            if nsteps >= 2 and self.fuses_next_first():
+            if nsteps >= 2 and self.chain and (self.fuses_next_first() or self._defers_scale()):
                 g_first, g_mid, g_last = self._chain(virial)
                 g_first.replay()
                 for _ in range(nsteps - 2):

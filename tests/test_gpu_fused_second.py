@@ -7,7 +7,8 @@ global tables (small launch) and on the persistent LDS-table kernel (more
 than 256 tiles). Graph runs of two or more steps also let the force pass do
 the next step's first half (the NPT replica's stays in its own kernel): the
 same bits as runs that do not, across list rebuilds and runs of 1, 2 and more
-steps. The LJ global-list pass with the second half fused in
+steps, and the Berendsen replicas' scaling left to the next step's first half
+equals the barostat's own. The LJ global-list pass with the second half fused in
 (sim_lj_force_second) equals k_lj followed by the separate kernel, bit for
 bit, on triclinic argon cells packed the same way.
 """
@@ -48,6 +49,7 @@ def _engine(setfl, sides, fused, skin=1.0, chain=True):
     if not chain:
         ff.fuses_next_first = lambda e: False
     eng.force_field = ff
+    eng.chain = chain
     eng.setup()
     return eng
 
@@ -84,7 +86,8 @@ def test_next_first_half_chain_bitwise(setfl, sides):
     b.run(2, graph=False)
     torch.cuda.synchronize()
     assert int((a.nbuild - nb0).min()) > 0, "no list rebuild in the compared steps"
-    for name in ("pos4", "vel", "vel_half", "force", "posq", "pos_last", "maxd2", "nbuild"):
+    for name in ("pos4", "vel", "vel_half", "force", "posq", "pos_last", "maxd2", "nbuild",
+                 "reps"):
         x, y = getattr(a, name), getattr(b, name)
         assert torch.equal(x, y), f"{name} differs"
     np.testing.assert_array_equal(a.thermo_host(), b.thermo_host())
@@ -122,3 +125,24 @@ def test_lj_fused_second_half_bitwise():
     ta, tb = a.thermo_host(), b.thermo_host()
     np.testing.assert_array_equal(ta, tb)
     assert np.all(np.isfinite(ta))
+
+
+def test_lj_npt_chain_bitwise():
+    """Graph runs that leave each Berendsen step's scaling of the atoms to the
+    next step's first half equal unchained runs, bit for bit, LJ global list."""
+    a = _lj_engine(True)
+    b = _lj_engine(True)
+    b.chain = False
+    print(f"[case] LJ Berendsen chain: {a.N} atoms", flush=True)
+    assert a._defers_scale()
+    for n in (2, 1, 25, 3):
+        a.run(n, graph=True)
+        b.run(n, graph=True)
+    a.run(2, graph=False)
+    b.run(2, graph=False)
+    torch.cuda.synchronize()
+    for name in ("pos4", "vel", "vel_half", "force", "posq", "pos_last", "maxd2", "nbuild",
+                 "reps"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert torch.equal(x, y), f"{name} differs"
+    np.testing.assert_array_equal(a.thermo_host(), b.thermo_host())
